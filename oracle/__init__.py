@@ -1,0 +1,18 @@
+"""CPU oracle for STile's hybrid sparse-tile SpMM / SDDMM.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this package, and only as the checker
+or the timed CPU baseline; the product (stile_amd/) never imports it.
+
+Contents
+  ops.py         FP64 SpMM/SDDMM, the reference's own FP32 CPU path
+                 (scipy csr.astype(float32).dot(B), MY_sparse/my_fuse_formats.py:4194-4197),
+                 error metrics.
+  formats.py     restatement of the reference's integer format construction
+                 (MY_sparse/gen_formats_v2.py, my_branch_and_bound.py,
+                 my_fuse_formats.py packers), following the reference text.
+  cost_model.py  restatement of the analytic cost model (MY_sparse/my_cost_model.py).
+
+Each function's docstring cites the reference file:line it follows and names
+the check that pins it (tests/test_oracle_*.py).
+"""

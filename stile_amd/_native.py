@@ -1,0 +1,340 @@
+"""ctypes binding of libstile_hip.so (include/stile.h).
+
+Tensors cross the boundary as raw device pointers; the current torch stream is
+passed as the HIP stream.  Loading fails loudly when the library is missing:
+there is no CPU fallback for any kernel.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+STILE_BLK_F32, STILE_BLK_BF16, STILE_BLK_F16 = 0, 1, 2
+_BLK_NAMES = {"float32": STILE_BLK_F32, "fp32": STILE_BLK_F32, "f32": STILE_BLK_F32,
+              "bfloat16": STILE_BLK_BF16, "bf16": STILE_BLK_BF16,
+              "float16": STILE_BLK_F16, "fp16": STILE_BLK_F16, "f16": STILE_BLK_F16}
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+class StileEllGroup(ctypes.Structure):
+    _fields_ = [("m_blk", ctypes.c_int32), ("nnz_cols", ctypes.c_int32),
+                ("num_tiles", ctypes.c_int32), ("real_atomic", ctypes.c_int32),
+                ("a", _f32p), ("indices_k", _i32p), ("indices_reordered_i", _i32p)]
+
+
+class StileTcGroup(ctypes.Structure):
+    _fields_ = [("group_size", ctypes.c_int32), ("num_windows", ctypes.c_int32),
+                ("nnzb", ctypes.c_int32), ("rows_2d", ctypes.c_int32),
+                ("k_notsorted", ctypes.c_int32), ("real_atomic", ctypes.c_int32),
+                ("a", _f32p), ("i_indices", _i32p), ("indptr", _i32p), ("indices", _i32p)]
+
+
+class StileSpmmPack(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int32), ("K", ctypes.c_int32),
+                ("num_ell", ctypes.c_int32), ("num_tc", ctypes.c_int32),
+                ("ell", ctypes.POINTER(StileEllGroup)), ("tc", ctypes.POINTER(StileTcGroup))]
+
+
+class StileSddmm1DGroup(ctypes.Structure):
+    _fields_ = [("len", ctypes.c_int32), ("mid_idx", _i32p), ("j_idx", _i32p), ("remap", _i32p)]
+
+
+class StileSddmmTcGroup(ctypes.Structure):
+    _fields_ = [("nnzb", ctypes.c_int32), ("blk_k", ctypes.c_int32),
+                ("i_indices", _i32p), ("j_indices", _i32p), ("indptr", _i32p),
+                ("indices_dense", _i32p), ("indices", _i32p)]
+
+
+class StileSddmmPack(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int32), ("N", ctypes.c_int32), ("nnz", ctypes.c_int32),
+                ("num_1d", ctypes.c_int32), ("num_tc", ctypes.c_int32),
+                ("g1d", ctypes.POINTER(StileSddmm1DGroup)),
+                ("tc", ctypes.POINTER(StileSddmmTcGroup))]
+
+
+class StilePlanInfo(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("blk_dtype", ctypes.c_int32),
+                ("M", ctypes.c_int32), ("K", ctypes.c_int32),
+                ("num_windows", ctypes.c_int32), ("num_chunks", ctypes.c_int32),
+                ("block_nnz", ctypes.c_int64), ("sparse_nnz", ctypes.c_int64),
+                ("device_bytes", ctypes.c_int64), ("hub_rows", ctypes.c_int32),
+                ("num_split", ctypes.c_int32)]
+
+
+# Every symbol of include/stile.h with its ctypes signature.
+_SIGNATURES = {
+    "stile_csr_spmm_f32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                          ctypes.c_int64, ctypes.c_void_p]),
+    "stile_csr_sddmm_f32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                           ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "stile_plan_create_spmm": (ctypes.c_int, [ctypes.POINTER(StileSpmmPack), ctypes.c_int32,
+                                              ctypes.POINTER(ctypes.c_void_p)]),
+    "stile_plan_create_spmm_batched": (ctypes.c_int, [ctypes.POINTER(StileSpmmPack), ctypes.c_int32,
+                                                      ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "stile_plan_create_sddmm": (ctypes.c_int, [ctypes.POINTER(StileSddmmPack), ctypes.c_int32,
+                                               ctypes.POINTER(ctypes.c_void_p)]),
+    "stile_plan_create_sddmm_batched": (ctypes.c_int, [ctypes.POINTER(StileSddmmPack), ctypes.c_int32,
+                                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "stile_spmm_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                      ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+    "stile_sddmm_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                       ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                       ctypes.c_void_p, ctypes.c_void_p]),
+    "stile_plan_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(StilePlanInfo)]),
+    "stile_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "stile_last_error": (ctypes.c_char_p, []),
+    "stile_abi_version": (ctypes.c_int32, []),
+}
+
+_lib = None
+
+
+class StileError(RuntimeError):
+    pass
+
+
+def lib_path():
+    return _build.LIB_PATH
+
+
+def load():
+    """Load libstile_hip.so (built in-tree by stile_amd.build); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise StileError(f"{path} is missing: build it with `python -m stile_amd.build` "
+                         "(there is no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().stile_last_error().decode(errors="replace")
+        raise StileError(f"libstile_hip error {rc}: {msg}")
+
+
+def _ptr(arr, ctype):
+    return arr.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _stream(t=None):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device if t is not None else None).cuda_stream)
+
+
+def blk_dtype_code(dtype):
+    if isinstance(dtype, int):
+        return dtype
+    key = str(dtype).replace("torch.", "")
+    if key not in _BLK_NAMES:
+        raise ValueError(f"unknown block dtype {dtype!r}")
+    return _BLK_NAMES[key]
+
+
+def _c_i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _c_f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ---------------------------------------------------------------------------
+# Plain CSR kernels.
+# ---------------------------------------------------------------------------
+
+def _dev_check(*ts):
+    for t in ts:
+        if not t.is_cuda:
+            raise StileError("libstile_hip kernels take device tensors")
+
+
+def csr_spmm_f32(indptr, indices, vals, B, out=None):
+    """C = A @ B for CSR A (int32 indptr/indices, float32 vals) on the device."""
+    import torch
+    _dev_check(indptr, indices, vals, B)
+    M = indptr.numel() - 1
+    K, N = B.shape
+    if B.dtype != torch.float32 or vals.dtype != torch.float32:
+        raise StileError("csr_spmm_f32 takes float32 values and B")
+    if B.stride(1) != 1:
+        B = B.contiguous()
+    C = out if out is not None else torch.empty((M, N), device=B.device, dtype=torch.float32)
+    _check(load().stile_csr_spmm_f32(M, K, N, indptr.data_ptr(), indices.data_ptr(), vals.data_ptr(),
+                                     B.data_ptr(), B.stride(0), C.data_ptr(), C.stride(0), _stream(B)))
+    return C
+
+
+def csr_sddmm_f32(indptr, indices, X, Y, out=None):
+    """out[p] = <X[row(p)], Y[indices[p]]> over the CSR pattern, on the device."""
+    import torch
+    _dev_check(indptr, indices, X, Y)
+    M = indptr.numel() - 1
+    nnz = indices.numel()
+    Kf = X.shape[1]
+    if X.stride(1) != 1:
+        X = X.contiguous()
+    if Y.stride(1) != 1:
+        Y = Y.contiguous()
+    o = out if out is not None else torch.empty(nnz, device=X.device, dtype=torch.float32)
+    _check(load().stile_csr_sddmm_f32(M, Y.shape[0], nnz, Kf, indptr.data_ptr(), indices.data_ptr(),
+                                      X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0),
+                                      o.data_ptr(), _stream(X)))
+    return o
+
+
+# ---------------------------------------------------------------------------
+# Plans.
+# ---------------------------------------------------------------------------
+
+class _PackArrays:
+    """Keeps the numpy buffers of a ctypes pack alive while the plan is built."""
+
+    def __init__(self):
+        self.keep = []
+
+    def i32(self, a):
+        a = _c_i32(a)
+        self.keep.append(a)
+        return _ptr(a, ctypes.c_int32)
+
+    def f32(self, a):
+        a = _c_f32(a)
+        self.keep.append(a)
+        return _ptr(a, ctypes.c_float)
+
+
+def _spmm_pack_struct(pack, arrays):
+    """pack: dict with M, K, ell (list of dicts), tc (list of dicts); Appendix-A layouts."""
+    ells = [StileEllGroup(m_blk=g["m_blk"], nnz_cols=g["nnz_cols"], num_tiles=g["num_tiles"],
+                          real_atomic=int(bool(g.get("real_atomic", False))),
+                          a=arrays.f32(g["a"]), indices_k=arrays.i32(g["indices_k"]),
+                          indices_reordered_i=arrays.i32(g["indices_reordered_i"]))
+            for g in pack["ell"]]
+    tcs = [StileTcGroup(group_size=g["group_size"], num_windows=g["num_windows"], nnzb=g["nnzb"],
+                        rows_2d=int(bool(g["rows_2d"])), k_notsorted=int(bool(g.get("k_notsorted", False))),
+                        real_atomic=int(bool(g.get("real_atomic", False))),
+                        a=arrays.f32(g["a"]), i_indices=arrays.i32(g["i_indices"]),
+                        indptr=arrays.i32(g["indptr"]), indices=arrays.i32(g["indices"]))
+           for g in pack["tc"]]
+    ell_arr = (StileEllGroup * max(len(ells), 1))(*ells)
+    tc_arr = (StileTcGroup * max(len(tcs), 1))(*tcs)
+    arrays.keep += [ell_arr, tc_arr]
+    return StileSpmmPack(M=pack["M"], K=pack["K"], num_ell=len(ells), num_tc=len(tcs),
+                         ell=ctypes.cast(ell_arr, ctypes.POINTER(StileEllGroup)),
+                         tc=ctypes.cast(tc_arr, ctypes.POINTER(StileTcGroup)))
+
+
+def _sddmm_pack_struct(pack, arrays):
+    g1 = [StileSddmm1DGroup(len=len(g["mid_idx"]), mid_idx=arrays.i32(g["mid_idx"]),
+                            j_idx=arrays.i32(g["j_idx"]), remap=arrays.i32(g["remap"]))
+          for g in pack["g1d"]]
+    tcs = [StileSddmmTcGroup(nnzb=g["nnzb"], blk_k=g.get("blk_k", 16),
+                             i_indices=arrays.i32(g["i_indices"]), j_indices=arrays.i32(g["j_indices"]),
+                             indptr=arrays.i32(g["indptr"]), indices_dense=arrays.i32(g["indices_dense"]),
+                             indices=arrays.i32(g["indices"]))
+           for g in pack["tc"]]
+    g1_arr = (StileSddmm1DGroup * max(len(g1), 1))(*g1)
+    tc_arr = (StileSddmmTcGroup * max(len(tcs), 1))(*tcs)
+    arrays.keep += [g1_arr, tc_arr]
+    return StileSddmmPack(M=pack["M"], N=pack["N"], nnz=pack["nnz"], num_1d=len(g1), num_tc=len(tcs),
+                          g1d=ctypes.cast(g1_arr, ctypes.POINTER(StileSddmm1DGroup)),
+                          tc=ctypes.cast(tc_arr, ctypes.POINTER(StileSddmmTcGroup)))
+
+
+class Plan:
+    """Owner of a StilePlan (device-resident tile buffers of one tile set)."""
+
+    def __init__(self, handle, kind, M, K, nnz=0):
+        self._h = handle
+        self.kind = kind
+        self.M, self.K, self.nnz = M, K, nnz
+
+    @classmethod
+    def spmm(cls, packs, blk_dtype="bfloat16"):
+        """Build from one pack dict or a list of them (block-diagonal batch)."""
+        lib = load()
+        packs = packs if isinstance(packs, (list, tuple)) else [packs]
+        arrays = _PackArrays()
+        structs = (StileSpmmPack * len(packs))(*[_spmm_pack_struct(p, arrays) for p in packs])
+        h = ctypes.c_void_p()
+        _check(lib.stile_plan_create_spmm_batched(structs, len(packs), blk_dtype_code(blk_dtype),
+                                                  ctypes.byref(h)))
+        return cls(h, "spmm", sum(p["M"] for p in packs), sum(p["K"] for p in packs))
+
+    @classmethod
+    def sddmm(cls, packs, blk_dtype="float32"):
+        lib = load()
+        packs = packs if isinstance(packs, (list, tuple)) else [packs]
+        arrays = _PackArrays()
+        structs = (StileSddmmPack * len(packs))(*[_sddmm_pack_struct(p, arrays) for p in packs])
+        h = ctypes.c_void_p()
+        _check(lib.stile_plan_create_sddmm_batched(structs, len(packs), blk_dtype_code(blk_dtype),
+                                                   ctypes.byref(h)))
+        return cls(h, "sddmm", sum(p["M"] for p in packs), sum(p["N"] for p in packs),
+                   sum(p["nnz"] for p in packs))
+
+    def info(self):
+        inf = StilePlanInfo()
+        _check(load().stile_plan_info(self._h, ctypes.byref(inf)))
+        return {f: getattr(inf, f) for f, _ in StilePlanInfo._fields_}
+
+    def run_spmm(self, B, out=None):
+        import torch
+        if self.kind != "spmm":
+            raise StileError("not an SpMM plan")
+        _dev_check(B)
+        if B.dtype != torch.float32:
+            raise StileError("B must be float32")
+        if B.shape[0] != self.K:
+            raise StileError(f"B has {B.shape[0]} rows, the plan's A has {self.K} columns")
+        if B.stride(1) != 1:
+            B = B.contiguous()
+        N = B.shape[1]
+        C = out if out is not None else torch.empty((self.M, N), device=B.device, dtype=torch.float32)
+        _check(load().stile_spmm_run(self._h, B.data_ptr(), B.stride(0), N, C.data_ptr(), C.stride(0),
+                                     _stream(B)))
+        return C
+
+    def run_sddmm(self, X, Y, out=None):
+        import torch
+        if self.kind != "sddmm":
+            raise StileError("not an SDDMM plan")
+        _dev_check(X, Y)
+        if X.shape[0] != self.M or Y.shape[0] != self.K or X.shape[1] != Y.shape[1]:
+            raise StileError("X/Y shapes do not match the plan")
+        X = X.contiguous() if X.stride(1) != 1 else X
+        Y = Y.contiguous() if Y.stride(1) != 1 else Y
+        o = out if out is not None else torch.empty(self.nnz, device=X.device, dtype=torch.float32)
+        _check(load().stile_sddmm_run(self._h, X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0),
+                                      X.shape[1], o.data_ptr(), _stream(X)))
+        return o
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            _check(load().stile_plan_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            if self._h is not None and self._h.value and _lib is not None:
+                _lib.stile_plan_destroy(self._h)
+        except Exception:
+            pass
+        self._h = None

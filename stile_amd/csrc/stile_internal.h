@@ -1,0 +1,80 @@
+// Internal declarations shared by the kernel and plan translation units of
+// libstile_hip.so.  Not part of the C ABI (include/stile.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace stile {
+
+// Hub rows: a row whose sparse (non-MFMA) part has more than this many
+// nonzeros is summed with compensated (Neumaier) FP32 arithmetic.
+constexpr int kHubNnz = 4096;
+
+// Device view of a hybrid SpMM plan (window-major execution layout).
+//
+// The rows of C are partitioned into 16-row windows (the reference's TC row
+// windows, my_branch_and_bound.py:238-389, extended to every row).  Window w
+// owns output rows win_rows[16w .. 16w+15] (-1 = empty slot).  Its dense
+// blocks are re-cut into chunks of 32 columns ("k-slots"):
+//   chunk c in tc_ptr[w] .. tc_ptr[w+1]
+//   tc_cols [c][g][e]       column of A (row of B) for k-slot k = 4e + g
+//   tc_vals [c][lane][e]    A[row lane%16][k-slot 4e + lane/16]
+// so that lane l of a wave fetches its MFMA A operand with one 16-byte
+// (bf16) or 32-byte (fp32) load and its 8 B-row ids with two int4 loads.
+// The FP32 part (ELL/1D tiles) is a CSR over the window slots:
+//   res_ptr[16w + r] .. res_ptr[16w + r + 1] -> (res_cols, res_vals).
+struct SpmmPlanDev {
+  int32_t M, K, W;
+  int32_t nchunks;
+  int32_t blk_dtype;
+  const int32_t* win_rows;   // [W*16]
+  const int32_t* tc_ptr;     // [W+1]
+  const int32_t* tc_cols;    // [nchunks*32]
+  const void* tc_vals;       // [nchunks*64*8] bf16/f16/f32
+  const float* tc_scale;     // [nchunks] (f16 only, else nullptr)
+  const int32_t* res_ptr;    // [W*16+1]
+  const int32_t* res_cols;
+  const float* res_vals;
+};
+
+// Device view of a hybrid SDDMM plan.
+//   1D part: n1d entries (row, col, out) computed as FP32 dot products.
+//   Block part: nblk 16x16 output blocks: rows blk_rows[b][16],
+//   cols blk_cols[b][16]; sampled entries s in blk_ptr[b]..blk_ptr[b+1]
+//   write out[blk_out[s]] = block[blk_pos[s] >> 4][blk_pos[s] & 15].
+struct SddmmPlanDev {
+  int32_t M, N, nnz;
+  int32_t n1d, nblk;
+  int32_t blk_dtype;
+  const int32_t* e_row;
+  const int32_t* e_col;
+  const int32_t* e_out;
+  const int32_t* blk_rows;
+  const int32_t* blk_cols;
+  const int32_t* blk_ptr;
+  const int32_t* blk_pos;
+  const int32_t* blk_out;
+};
+
+// Kernel launchers (stile_kernels.hip).  All return hipError_t of the launch.
+hipError_t launch_csr_spmm_f32(int M, int N, const int32_t* indptr,
+                               const int32_t* indices, const float* vals,
+                               const float* B, int64_t ldb, float* C,
+                               int64_t ldc, hipStream_t s);
+hipError_t launch_csr_sddmm_f32(int M, int nnz, int Kf, const int32_t* indptr,
+                                const int32_t* indices, const float* X,
+                                int64_t ldx, const float* Y, int64_t ldy,
+                                float* out, hipStream_t s);
+hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb,
+                           int N, float* C, int64_t ldc, hipStream_t s);
+hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,
+                            int64_t ldx, const float* Y, int64_t ldy, int Kf,
+                            float* out, hipStream_t s);
+
+// Zero-fill kernels (used instead of hipMemsetAsync so that every *_run call
+// records only kernel nodes under HIP graph capture).
+hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t s);
+hipError_t launch_fill_zero_2d(float* p, int rows, int cols, int64_t ld, hipStream_t s);
+
+}  // namespace stile

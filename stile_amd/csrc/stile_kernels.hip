@@ -1,0 +1,570 @@
+// HIP kernels of libstile_hip.so for MI355X (gfx950, CDNA4, wave64).
+//
+// Four kernel templates:
+//   csr_spmm_f32     plain CSR SpMM (the comparator; reference csrmm template,
+//                    MY_sparse/my_fuse_formats.py:1106-1152)
+//   csr_sddmm_f32    plain per-nonzero SDDMM (reference golden u_dot_v,
+//                    my_fuse_formats.py:4331-4341)
+//   hyb_spmm         hybrid fused SpMM: MFMA on dense 16-row blocks + FP32
+//                    sparse part, one wave per (16-row window, 64 columns),
+//                    every C row written once (replaces the fused ELL +
+//                    TensorCore templates, my_fuse_formats.py:1162-1291,
+//                    1444-1799, 2426-2465)
+//   hyb_sddmm        hybrid fused SDDMM: 1D entries as FP32 wave-segment dot
+//                    products + dense 16x16 blocks on MFMA with sampled
+//                    scatter (my_fuse_formats.py:2263-2317, 2324-2415,
+//                    2519-2560)
+#include "stile_internal.h"
+
+#include <hip/hip_bf16.h>
+
+namespace stile {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float4 ld4(const float* p) {
+  return *reinterpret_cast<const float4*>(p);
+}
+__device__ __forceinline__ void st4(float* p, float4 v) {
+  *reinterpret_cast<float4*>(p) = v;
+}
+__device__ __forceinline__ float comp(const float4& v, int t) {
+  return t == 0 ? v.x : (t == 1 ? v.y : (t == 2 ? v.z : v.w));
+}
+
+// Neumaier compensated accumulation: s + c carries the sum of every x added.
+// Contraction is off so (s - t) + x is evaluated as written.
+__device__ __forceinline__ void neu_add(float& s, float& c, float x) {
+#pragma clang fp contract(off)
+  float t = s + x;
+  if (fabsf(s) >= fabsf(x))
+    c += (s - t) + x;
+  else
+    c += (x - t) + s;
+  s = t;
+}
+
+// Load 4 consecutive floats of row k of B starting at column col; columns at
+// or beyond N read as 0.  vec: B rows are 16-byte aligned and N % 4 == 0.
+__device__ __forceinline__ float4 load_b4(const float* __restrict__ B,
+                                          int64_t ldb, int k, int col, int N,
+                                          bool vec) {
+  const float* p = B + (int64_t)k * ldb + col;
+  if (vec) {
+    if (col < N) return ld4(p);
+    return make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  float4 r;
+  r.x = col < N ? p[0] : 0.f;
+  r.y = col + 1 < N ? p[1] : 0.f;
+  r.z = col + 2 < N ? p[2] : 0.f;
+  r.w = col + 3 < N ? p[3] : 0.f;
+  return r;
+}
+
+__device__ __forceinline__ void store_c4(float* __restrict__ C, int64_t ldc,
+                                         int row, int col, int N, bool vec,
+                                         float4 v) {
+  float* p = C + (int64_t)row * ldc + col;
+  if (vec) {
+    if (col < N) st4(p, v);
+    return;
+  }
+  if (col < N) p[0] = v.x;
+  if (col + 1 < N) p[1] = v.y;
+  if (col + 2 < N) p[2] = v.z;
+  if (col + 3 < N) p[3] = v.w;
+}
+
+// ---------------------------------------------------------------------------
+// Plain CSR SpMM.  A segment of LPR lanes owns one row and CPL groups of
+// 4*LPR consecutive columns (one float4 per lane per group); a wave holds
+// 64/LPR rows.  Grid: (row groups) x (column blocks of 4*LPR*CPL).
+// ---------------------------------------------------------------------------
+template <int LPR, int CPL>
+__global__ __launch_bounds__(256) void csr_spmm_f32_kernel(
+    int M, int N, const int32_t* __restrict__ indptr,
+    const int32_t* __restrict__ indices, const float* __restrict__ vals,
+    const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
+    int64_t ldc, bool vec) {
+  constexpr int RPW = 64 / LPR;
+  const int lane = threadIdx.x & 63;
+  const int sub = lane % LPR;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int row = wave * RPW + lane / LPR;
+  if (row >= M) return;
+  const int col0 = blockIdx.y * (4 * LPR * CPL) + 4 * sub;
+  const int p0 = indptr[row], p1 = indptr[row + 1];
+
+  float4 acc[CPL];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+  if (p1 - p0 <= kHubNnz) {
+    int p = p0;
+    for (; p + 4 <= p1; p += 4) {
+      const int k0 = indices[p], k1 = indices[p + 1], k2 = indices[p + 2],
+                k3 = indices[p + 3];
+      const float v0 = vals[p], v1 = vals[p + 1], v2 = vals[p + 2],
+                  v3 = vals[p + 3];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) {
+        const int col = col0 + q * 4 * LPR;
+        const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
+        const float4 b1 = load_b4(B, ldb, k1, col, N, vec);
+        const float4 b2 = load_b4(B, ldb, k2, col, N, vec);
+        const float4 b3 = load_b4(B, ldb, k3, col, N, vec);
+        acc[q].x += v0 * b0.x; acc[q].y += v0 * b0.y; acc[q].z += v0 * b0.z; acc[q].w += v0 * b0.w;
+        acc[q].x += v1 * b1.x; acc[q].y += v1 * b1.y; acc[q].z += v1 * b1.z; acc[q].w += v1 * b1.w;
+        acc[q].x += v2 * b2.x; acc[q].y += v2 * b2.y; acc[q].z += v2 * b2.z; acc[q].w += v2 * b2.w;
+        acc[q].x += v3 * b3.x; acc[q].y += v3 * b3.y; acc[q].z += v3 * b3.z; acc[q].w += v3 * b3.w;
+      }
+    }
+    for (; p < p1; ++p) {
+      const int k = indices[p];
+      const float v = vals[p];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) {
+        const float4 b = load_b4(B, ldb, k, col0 + q * 4 * LPR, N, vec);
+        acc[q].x += v * b.x; acc[q].y += v * b.y; acc[q].z += v * b.z; acc[q].w += v * b.w;
+      }
+    }
+  } else {
+    // Hub row: compensated FP32 summation.
+    float4 cc[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) cc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int p = p0; p < p1; ++p) {
+      const int k = indices[p];
+      const float v = vals[p];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) {
+        const float4 b = load_b4(B, ldb, k, col0 + q * 4 * LPR, N, vec);
+        neu_add(acc[q].x, cc[q].x, v * b.x);
+        neu_add(acc[q].y, cc[q].y, v * b.y);
+        neu_add(acc[q].z, cc[q].z, v * b.z);
+        neu_add(acc[q].w, cc[q].w, v * b.w);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      acc[q].x += cc[q].x; acc[q].y += cc[q].y; acc[q].z += cc[q].z; acc[q].w += cc[q].w;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < CPL; ++q)
+    store_c4(C, ldc, row, col0 + q * 4 * LPR, N, vec, acc[q]);
+}
+
+template <int LPR, int CPL>
+static hipError_t launch_csr_spmm_cfg(int M, int N, const int32_t* indptr,
+                                      const int32_t* indices,
+                                      const float* vals, const float* B,
+                                      int64_t ldb, float* C, int64_t ldc,
+                                      bool vec, hipStream_t s) {
+  constexpr int RPW = 64 / LPR;
+  const int waves = (M + RPW - 1) / RPW;
+  const int cb = 4 * LPR * CPL;
+  dim3 grid((waves + 3) / 4, (N + cb - 1) / cb);
+  hipLaunchKernelGGL((csr_spmm_f32_kernel<LPR, CPL>), grid, dim3(256), 0, s,
+                     M, N, indptr, indices, vals, B, ldb, C, ldc, vec);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_spmm_f32(int M, int N, const int32_t* indptr,
+                               const int32_t* indices, const float* vals,
+                               const float* B, int64_t ldb, float* C,
+                               int64_t ldc, hipStream_t s) {
+  if (M == 0 || N == 0) return hipSuccess;
+  const bool vec = (N % 4 == 0) && (ldb % 4 == 0) && (ldc % 4 == 0) &&
+                   ((reinterpret_cast<uintptr_t>(B) & 15) == 0) &&
+                   ((reinterpret_cast<uintptr_t>(C) & 15) == 0);
+  if (N <= 16)
+    return launch_csr_spmm_cfg<4, 1>(M, N, indptr, indices, vals, B, ldb, C, ldc, vec, s);
+  if (N <= 32)
+    return launch_csr_spmm_cfg<8, 1>(M, N, indptr, indices, vals, B, ldb, C, ldc, vec, s);
+  if (N <= 64)
+    return launch_csr_spmm_cfg<16, 1>(M, N, indptr, indices, vals, B, ldb, C, ldc, vec, s);
+  if (N <= 128)
+    return launch_csr_spmm_cfg<32, 1>(M, N, indptr, indices, vals, B, ldb, C, ldc, vec, s);
+  if (N <= 256)
+    return launch_csr_spmm_cfg<64, 1>(M, N, indptr, indices, vals, B, ldb, C, ldc, vec, s);
+  return launch_csr_spmm_cfg<64, 2>(M, N, indptr, indices, vals, B, ldb, C, ldc, vec, s);
+}
+
+// ---------------------------------------------------------------------------
+// Plain CSR SDDMM: a segment of LPE lanes owns one nonzero; each lane loads
+// float4 slices of X[row] and Y[col] and the segment reduces with xor
+// shuffles.  The row of each nonzero is found by binary search in indptr.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_row(const int32_t* __restrict__ indptr,
+                                        int M, int p) {
+  int lo = 0, hi = M;  // indptr[lo] <= p < indptr[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (indptr[mid] <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int LPE>
+__device__ __forceinline__ float segment_dot(const float* __restrict__ x,
+                                             const float* __restrict__ y,
+                                             int Kf, int sub, bool vec) {
+  float s = 0.f;
+  if (vec) {
+    for (int k = 4 * sub; k < Kf; k += 4 * LPE) {
+      const float4 a = ld4(x + k), b = ld4(y + k);
+      s += a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+    }
+  } else {
+    for (int k = sub; k < Kf; k += LPE) s += x[k] * y[k];
+  }
+#pragma unroll
+  for (int off = LPE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPE);
+  return s;
+}
+
+template <int LPE>
+__global__ __launch_bounds__(256) void csr_sddmm_f32_kernel(
+    int M, int nnz, int Kf, const int32_t* __restrict__ indptr,
+    const int32_t* __restrict__ indices, const float* __restrict__ X,
+    int64_t ldx, const float* __restrict__ Y, int64_t ldy,
+    float* __restrict__ out, bool vec) {
+  constexpr int EPW = 64 / LPE;
+  const int lane = threadIdx.x & 63;
+  const int sub = lane % LPE;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int p = wave * EPW + lane / LPE;
+  if (p >= nnz) return;
+  const int row = find_row(indptr, M, p);
+  const int col = indices[p];
+  const float s = segment_dot<LPE>(X + (int64_t)row * ldx,
+                                   Y + (int64_t)col * ldy, Kf, sub, vec);
+  if (sub == 0) out[p] = s;
+}
+
+hipError_t launch_csr_sddmm_f32(int M, int nnz, int Kf, const int32_t* indptr,
+                                const int32_t* indices, const float* X,
+                                int64_t ldx, const float* Y, int64_t ldy,
+                                float* out, hipStream_t s) {
+  if (nnz == 0) return hipSuccess;
+  const bool vec = (Kf % 4 == 0) && (ldx % 4 == 0) && (ldy % 4 == 0) &&
+                   ((reinterpret_cast<uintptr_t>(X) & 15) == 0) &&
+                   ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
+  int lpe = vec ? Kf / 4 : Kf;
+  if (lpe > 16) lpe = 16;
+  int l = 1;
+  while (l * 2 <= lpe) l *= 2;
+  lpe = l;
+  const int epw = 64 / lpe;
+  const int waves = (nnz + epw - 1) / epw;
+  dim3 grid((waves + 3) / 4);
+#define STILE_SDDMM_CASE(L)                                                  \
+  case L:                                                                    \
+    hipLaunchKernelGGL(csr_sddmm_f32_kernel<L>, grid, dim3(256), 0, s, M,    \
+                       nnz, Kf, indptr, indices, X, ldx, Y, ldy, out, vec); \
+    break;
+  switch (lpe) {
+    STILE_SDDMM_CASE(1)
+    STILE_SDDMM_CASE(2)
+    STILE_SDDMM_CASE(4)
+    STILE_SDDMM_CASE(8)
+    STILE_SDDMM_CASE(16)
+  }
+#undef STILE_SDDMM_CASE
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Hybrid fused SpMM.
+//
+// One wave = (window w, 64 output columns n0..n0+63).  Lane l = 16g + j owns
+// output columns n0 + 4j .. n0 + 4j + 3 of rows 4g .. 4g+3 of the window: the
+// four 16x16 MFMA tiles t = 0..3 of the wave hold column n0 + 4j + t in MFMA
+// column j, so every B load and every C store is a float4 and MFMA output
+// element (row 4g+i, column j) of tile t is exactly the lane's own element.
+//
+// Dense chunk (32 k-slots): k-slot 4e + g of lane l is column tc_cols[c][g][e]
+// of A; the lane loads B row tc_cols[c][g][e], columns n0+4j..+3, for e = 0..7.
+//   BF16: one v_mfma_f32_16x16x32_bf16 per tile (operand element e of lane l
+//         is k-slot 4e + g for both A and B; the pairing is what matters).
+//   FP32: eight v_mfma_f32_16x16x4_f32 per tile (MFMA e reduces k-slots
+//         4e + 0..3; lane group g supplies k-slot 4e + g).
+// Sparse part: the lane adds v * B[col][n0+4j..+3] for every FP32 nonzero of
+// its 4 rows (compensated when a row has more than kHubNnz of them).
+// ---------------------------------------------------------------------------
+template <int MODE>
+__device__ __forceinline__ void hyb_spmm_window(const SpmmPlanDev& p,
+                                                const float* __restrict__ B,
+                                                int64_t ldb, int N,
+                                                float* __restrict__ C,
+                                                int64_t ldc, int w, int n0,
+                                                bool vec) {
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const int j = lane & 15;
+  const int col = n0 + 4 * j;
+
+  f32x4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int c0 = p.tc_ptr[w], c1 = p.tc_ptr[w + 1];
+  for (int c = c0; c < c1; ++c) {
+    const int4* cp = reinterpret_cast<const int4*>(p.tc_cols + (int64_t)c * 32 + g * 8);
+    const int4 ka = cp[0], kb = cp[1];
+    float4 b[8];
+    b[0] = load_b4(B, ldb, ka.x, col, N, vec);
+    b[1] = load_b4(B, ldb, ka.y, col, N, vec);
+    b[2] = load_b4(B, ldb, ka.z, col, N, vec);
+    b[3] = load_b4(B, ldb, ka.w, col, N, vec);
+    b[4] = load_b4(B, ldb, kb.x, col, N, vec);
+    b[5] = load_b4(B, ldb, kb.y, col, N, vec);
+    b[6] = load_b4(B, ldb, kb.z, col, N, vec);
+    b[7] = load_b4(B, ldb, kb.w, col, N, vec);
+    if constexpr (MODE == 1) {
+      const bf16x8 a = reinterpret_cast<const bf16x8*>(p.tc_vals)[(int64_t)c * 64 + lane];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bf16x8 bt;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bt[e] = (__bf16)comp(b[e], t);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bt, acc[t], 0, 0, 0);
+      }
+    } else if constexpr (MODE == 2) {
+      const f16x8 a = reinterpret_cast<const f16x8*>(p.tc_vals)[(int64_t)c * 64 + lane];
+      const float sc = p.tc_scale[c];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        f16x8 bt;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bt[e] = (_Float16)comp(b[e], t);
+        f32x4 tmp = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+            a, bt, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        acc[t] += sc * tmp;
+      }
+    } else {
+      const float* ap = reinterpret_cast<const float*>(p.tc_vals) + ((int64_t)c * 64 + lane) * 8;
+      const float4 a0 = ld4(ap), a1 = ld4(ap + 4);
+      const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], comp(b[e], t), acc[t], 0, 0, 0);
+      }
+    }
+  }
+
+  // FP32 sparse part + store.
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int slot = w * 16 + g * 4 + i;
+    const int r0 = p.res_ptr[slot], r1 = p.res_ptr[slot + 1];
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (r1 - r0 <= kHubNnz) {
+      int q = r0;
+      for (; q + 2 <= r1; q += 2) {
+        const int k0 = p.res_cols[q], k1 = p.res_cols[q + 1];
+        const float v0 = p.res_vals[q], v1 = p.res_vals[q + 1];
+        const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
+        const float4 b1 = load_b4(B, ldb, k1, col, N, vec);
+        s.x += v0 * b0.x; s.y += v0 * b0.y; s.z += v0 * b0.z; s.w += v0 * b0.w;
+        s.x += v1 * b1.x; s.y += v1 * b1.y; s.z += v1 * b1.z; s.w += v1 * b1.w;
+      }
+      if (q < r1) {
+        const int k0 = p.res_cols[q];
+        const float v0 = p.res_vals[q];
+        const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
+        s.x += v0 * b0.x; s.y += v0 * b0.y; s.z += v0 * b0.z; s.w += v0 * b0.w;
+      }
+    } else {
+      float4 cc = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int q = r0; q < r1; ++q) {
+        const int k0 = p.res_cols[q];
+        const float v0 = p.res_vals[q];
+        const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
+        neu_add(s.x, cc.x, v0 * b0.x);
+        neu_add(s.y, cc.y, v0 * b0.y);
+        neu_add(s.z, cc.z, v0 * b0.z);
+        neu_add(s.w, cc.w, v0 * b0.w);
+      }
+      s.x += cc.x; s.y += cc.y; s.z += cc.z; s.w += cc.w;
+    }
+    const int row = p.win_rows[slot];
+    if (row >= 0) {
+      float4 o;
+      o.x = acc[0][i] + s.x;
+      o.y = acc[1][i] + s.y;
+      o.z = acc[2][i] + s.z;
+      o.w = acc[3][i] + s.w;
+      store_c4(C, ldc, row, col, N, vec, o);
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void hyb_spmm_kernel(SpmmPlanDev p,
+                                                       const float* __restrict__ B,
+                                                       int64_t ldb, int N,
+                                                       float* __restrict__ C,
+                                                       int64_t ldc, int nC,
+                                                       bool vec) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int w = wave / nC;
+  if (w >= p.W) return;
+  const int n0 = (wave - w * nC) * 64;
+  hyb_spmm_window<MODE>(p, B, ldb, N, C, ldc, w, n0, vec);
+}
+
+hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb,
+                           int N, float* C, int64_t ldc, hipStream_t s) {
+  if (p.W == 0 || N == 0) return hipSuccess;
+  const bool vec = (N % 4 == 0) && (ldb % 4 == 0) && (ldc % 4 == 0) &&
+                   ((reinterpret_cast<uintptr_t>(B) & 15) == 0) &&
+                   ((reinterpret_cast<uintptr_t>(C) & 15) == 0);
+  const int nC = (N + 63) / 64;
+  const int64_t waves = (int64_t)p.W * nC;
+  dim3 grid((unsigned)((waves + 3) / 4));
+  switch (p.blk_dtype) {
+    case 1:
+      hipLaunchKernelGGL(hyb_spmm_kernel<1>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, vec);
+      break;
+    case 2:
+      hipLaunchKernelGGL(hyb_spmm_kernel<2>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, vec);
+      break;
+    default:
+      hipLaunchKernelGGL(hyb_spmm_kernel<0>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, vec);
+      break;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Hybrid fused SDDMM.  Waves [0, waves1d) run the 1D entries (segments of LPE
+// lanes, one entry each); the remaining waves each run one dense 16x16 block:
+// lane l = 16g + j loads float4 slices of X[rows[j]] and Y[cols[j]] at
+// features kb + 4g .. kb + 4g + 3 and feeds component e to MFMA e
+// (v_mfma_f32_16x16x4_f32, so k-slot g of MFMA e is feature kb + 4g + e).
+// The 16x16 result goes through LDS to the sampled scatter.
+// ---------------------------------------------------------------------------
+template <int LPE>
+__global__ __launch_bounds__(256) void hyb_sddmm_kernel(SddmmPlanDev p,
+                                                        const float* __restrict__ X,
+                                                        int64_t ldx,
+                                                        const float* __restrict__ Y,
+                                                        int64_t ldy, int Kf,
+                                                        float* __restrict__ out,
+                                                        int waves1d, bool vec) {
+  __shared__ float blk[4][256];
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const int wave = blockIdx.x * 4 + wib;
+  if (wave < waves1d) {
+    constexpr int EPW = 64 / LPE;
+    const int sub = lane % LPE;
+    const int e = wave * EPW + lane / LPE;
+    if (e >= p.n1d) return;
+    const int row = p.e_row[e], col = p.e_col[e];
+    const float s = segment_dot<LPE>(X + (int64_t)row * ldx,
+                                     Y + (int64_t)col * ldy, Kf, sub, vec);
+    if (sub == 0) out[p.e_out[e]] = s;
+    return;
+  }
+  const int b = wave - waves1d;
+  if (b >= p.nblk) return;
+  const int g = lane >> 4, j = lane & 15;
+  const float* xr = X + (int64_t)p.blk_rows[b * 16 + j] * ldx;
+  const float* yr = Y + (int64_t)p.blk_cols[b * 16 + j] * ldy;
+  f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int kb = 0; kb < Kf; kb += 16) {
+    const int k = kb + 4 * g;
+    float4 xa, ya;
+    if (vec && k + 3 < Kf) {
+      xa = ld4(xr + k);
+      ya = ld4(yr + k);
+    } else {
+      xa.x = k < Kf ? xr[k] : 0.f;         ya.x = k < Kf ? yr[k] : 0.f;
+      xa.y = k + 1 < Kf ? xr[k + 1] : 0.f; ya.y = k + 1 < Kf ? yr[k + 1] : 0.f;
+      xa.z = k + 2 < Kf ? xr[k + 2] : 0.f; ya.z = k + 2 < Kf ? yr[k + 2] : 0.f;
+      xa.w = k + 3 < Kf ? xr[k + 3] : 0.f; ya.w = k + 3 < Kf ? yr[k + 3] : 0.f;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.x, ya.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.y, ya.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.z, ya.z, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.w, ya.w, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) blk[wib][(4 * g + i) * 16 + j] = acc[i];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int s0 = p.blk_ptr[b], s1 = p.blk_ptr[b + 1];
+  for (int s = s0 + lane; s < s1; s += 64) out[p.blk_out[s]] = blk[wib][p.blk_pos[s]];
+}
+
+hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,
+                            int64_t ldx, const float* Y, int64_t ldy, int Kf,
+                            float* out, hipStream_t s) {
+  const bool vec = (Kf % 4 == 0) && (ldx % 4 == 0) && (ldy % 4 == 0) &&
+                   ((reinterpret_cast<uintptr_t>(X) & 15) == 0) &&
+                   ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
+  int lpe = vec ? Kf / 4 : Kf;
+  if (lpe > 16) lpe = 16;
+  if (lpe < 1) lpe = 1;
+  int l = 1;
+  while (l * 2 <= lpe) l *= 2;
+  lpe = l;
+  const int epw = 64 / lpe;
+  const int waves1d = (p.n1d + epw - 1) / epw;
+  const int64_t waves = (int64_t)waves1d + p.nblk;
+  if (waves == 0) return hipSuccess;
+  dim3 grid((unsigned)((waves + 3) / 4));
+#define STILE_HSDDMM_CASE(L)                                                  \
+  case L:                                                                     \
+    hipLaunchKernelGGL(hyb_sddmm_kernel<L>, grid, dim3(256), 0, s, p, X, ldx, \
+                       Y, ldy, Kf, out, waves1d, vec);                        \
+    break;
+  switch (lpe) {
+    STILE_HSDDMM_CASE(1)
+    STILE_HSDDMM_CASE(2)
+    STILE_HSDDMM_CASE(4)
+    STILE_HSDDMM_CASE(8)
+    STILE_HSDDMM_CASE(16)
+  }
+#undef STILE_HSDDMM_CASE
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Zero fill.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void fill_zero_2d_kernel(float* __restrict__ p, int rows, int cols,
+                                                           int64_t ld) {
+  const int64_t n = (int64_t)rows * cols;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / cols, c = i - r * cols;
+    p[r * ld + c] = 0.f;
+  }
+}
+
+hipError_t launch_fill_zero_2d(float* p, int rows, int cols, int64_t ld, hipStream_t s) {
+  const int64_t n = (int64_t)rows * cols;
+  if (n == 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_zero_2d_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, rows, cols, ld);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (n <= INT32_MAX) return launch_fill_zero_2d(p, 1, (int)n, n, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace stile

@@ -1,0 +1,725 @@
+// Host side of libstile_hip.so: the C ABI of include/stile.h, validation of
+// the packed tile groups, and their translation into the window-major
+// execution layout the hybrid kernels consume (stile_internal.h).
+//
+// The packed groups follow the reference's argument lists
+// (MY_sparse/my_fuse_formats.py:3321-3537 for SpMM, :3544-3678 for SDDMM).
+// The reference hands them to one horizontally fused kernel whose ELL and
+// TC blocks race on shared output rows and resolve it with atomicAdd
+// (nvcc.py:192-441, C zeroed first, my_fuse_formats.py:4175).  Here every
+// product that lands on an output row is gathered, at plan time, into the
+// 16-row window that owns the row, so the kernel writes each row once.
+#include "stile_internal.h"
+#include "../../include/stile.h"
+
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+struct StilePlan {
+  int32_t kind = 0;  // 1 SpMM, 2 SDDMM
+  int32_t blk_dtype = 0;
+  void* dev = nullptr;  // one device allocation holding every plan array
+  size_t dev_bytes = 0;
+  int device = 0;
+  stile::SpmmPlanDev spmm{};
+  stile::SddmmPlanDev sddmm{};
+  StilePlanInfo info{};
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return STILE_E_HIP;
+}
+
+// Bump allocator over one device buffer; every array starts 256-byte aligned.
+struct Layout {
+  size_t bytes = 0;
+  size_t add(size_t n) {
+    size_t off = (bytes + 255) & ~size_t(255);
+    bytes = off + n;
+    return off;
+  }
+};
+
+struct HostBlob {
+  std::vector<unsigned char> data;
+  void put(size_t off, const void* src, size_t n) {
+    if (n) std::memcpy(data.data() + off, src, n);
+  }
+};
+
+int upload(StilePlan* plan, const Layout& lay, const HostBlob& blob) {
+  hipError_t e = hipGetDevice(&plan->device);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  plan->dev_bytes = std::max<size_t>(lay.bytes, 256);
+  e = hipMalloc(&plan->dev, plan->dev_bytes);
+  if (e != hipSuccess) {
+    plan->dev = nullptr;
+    return fail(STILE_E_NOMEM, std::string("hipMalloc of plan: ") + hipGetErrorString(e));
+  }
+  if (lay.bytes) {
+    e = hipMemcpy(plan->dev, blob.data.data(), lay.bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy of plan");
+  }
+  return STILE_OK;
+}
+
+template <class T>
+const T* dptr(const StilePlan* plan, size_t off) {
+  return reinterpret_cast<const T*>(static_cast<const unsigned char*>(plan->dev) + off);
+}
+
+uint16_t to_bf16_bits(float f) {
+  __hip_bfloat16 h = __float2bfloat16(f);  // round to nearest even
+  uint16_t bits;
+  std::memcpy(&bits, &h, 2);
+  return bits;
+}
+
+uint16_t to_f16_bits(float f) {
+  _Float16 h = (_Float16)f;
+  uint16_t bits;
+  std::memcpy(&bits, &h, 2);
+  return bits;
+}
+
+// ---------------------------------------------------------------------------
+// SpMM plan construction.
+// ---------------------------------------------------------------------------
+
+struct DenseCol {
+  int32_t col;
+  float v[16];  // values for window slots 0..15
+};
+
+struct SpmmBuild {
+  int32_t M = 0, K = 0;
+  // windows
+  std::vector<int32_t> win_rows;               // [W*16]
+  std::vector<std::vector<DenseCol>> win_cols;  // dense columns per window
+  std::vector<int32_t> row_win, row_slot;      // per output row
+  // sparse part, as (row, col, val) triples
+  std::vector<int32_t> s_row, s_col;
+  std::vector<float> s_val;
+  int64_t block_nnz = 0;
+};
+
+// Validate one pack and append its windows / products to the build with row
+// offset roff and column offset coff (block-diagonal batching).
+int add_spmm_pack(SpmmBuild& bd, const StileSpmmPack* pk, int32_t roff, int32_t coff) {
+  if (!pk) return fail(STILE_E_INVALID, "null pack");
+  if (pk->M < 0 || pk->K < 0) return fail(STILE_E_INVALID, "negative M or K");
+  if ((pk->num_ell > 0 && !pk->ell) || (pk->num_tc > 0 && !pk->tc) || pk->num_ell < 0 || pk->num_tc < 0)
+    return fail(STILE_E_INVALID, "group array missing");
+  const int32_t M = pk->M, K = pk->K;
+  const size_t base_w = bd.win_rows.size() / 16;
+
+  // 1. windows from the TC groups, keyed by their first real row.
+  std::unordered_map<int32_t, int32_t> key2win;  // first row -> window index
+  std::vector<int32_t> local_row_win(M, -1), local_row_slot(M, -1);
+  std::vector<std::vector<int32_t>> group_slot_win(pk->num_tc);
+  for (int gi = 0; gi < pk->num_tc; ++gi) {
+    const StileTcGroup& g = pk->tc[gi];
+    if (g.group_size <= 0 || g.group_size % 16 || g.num_windows < 0 || g.nnzb < 0)
+      return fail(STILE_E_INVALID, "TC group: bad group_size/num_windows/nnzb");
+    if (!g.indptr || !g.indices || !g.i_indices || (g.nnzb > 0 && !g.a))
+      return fail(STILE_E_INVALID, "TC group: null buffer");
+    if (g.k_notsorted && g.group_size != 16)
+      return fail(STILE_E_INVALID, "TC group: k_notsorted requires group_size 16");
+    if (g.indptr[0] != 0 || g.indptr[g.num_windows] != g.nnzb)
+      return fail(STILE_E_INVALID, "TC group: indptr does not span [0, nnzb]");
+    group_slot_win[gi].assign(g.num_windows, -1);
+    for (int s = 0; s < g.num_windows; ++s) {
+      if (g.indptr[s + 1] < g.indptr[s]) return fail(STILE_E_INVALID, "TC group: indptr not monotone");
+      int32_t rows[16];
+      if (g.rows_2d) {
+        int32_t last = -1;
+        for (int r = 0; r < 16; ++r) {
+          int32_t v = g.i_indices[(int64_t)s * 16 + r];
+          if (v < 0 || v > M) return fail(STILE_E_INVALID, "TC group: i_indices out of range");
+          // Padding repeats the window's last row; M is the sentinel row.
+          rows[r] = (r > 0 && v == last) || v >= M ? -1 : v;
+          if (rows[r] >= 0) last = v;
+        }
+      } else {
+        int32_t w = g.i_indices[s];
+        if (w < 0 || (int64_t)w * 16 >= std::max<int64_t>(M, 1))
+          return fail(STILE_E_INVALID, "TC group: window id out of range");
+        for (int r = 0; r < 16; ++r) rows[r] = (int64_t)w * 16 + r < M ? w * 16 + r : -1;
+      }
+      int32_t key = -1;
+      for (int r = 0; r < 16; ++r)
+        if (rows[r] >= 0) { key = rows[r]; break; }
+      if (key < 0) continue;  // a window with no real rows carries nothing
+      auto it = key2win.find(key);
+      int32_t win;
+      if (it == key2win.end()) {
+        win = (int32_t)(bd.win_rows.size() / 16);
+        key2win.emplace(key, win);
+        for (int r = 0; r < 16; ++r) {
+          if (rows[r] >= 0) {
+            if (local_row_win[rows[r]] >= 0)
+              return fail(STILE_E_INVALID, "TC groups: a row belongs to two different windows");
+            local_row_win[rows[r]] = win;
+            local_row_slot[rows[r]] = r;
+          }
+          bd.win_rows.push_back(rows[r] >= 0 ? rows[r] + roff : -1);
+        }
+        bd.win_cols.emplace_back();
+      } else {
+        win = it->second;
+        for (int r = 0; r < 16; ++r)
+          if (rows[r] >= 0 && (local_row_win[rows[r]] != win || local_row_slot[rows[r]] != r))
+            return fail(STILE_E_INVALID, "TC groups disagree on a window's row order");
+      }
+      group_slot_win[gi][s] = win;
+    }
+  }
+  // 2. rows no TC window owns: windows of 16 consecutive leftover rows.
+  {
+    std::vector<int32_t> left;
+    for (int32_t r = 0; r < M; ++r)
+      if (local_row_win[r] < 0) left.push_back(r);
+    for (size_t i = 0; i < left.size(); i += 16) {
+      int32_t win = (int32_t)(bd.win_rows.size() / 16);
+      for (size_t r = 0; r < 16; ++r) {
+        if (i + r < left.size()) {
+          local_row_win[left[i + r]] = win;
+          local_row_slot[left[i + r]] = (int32_t)r;
+          bd.win_rows.push_back(left[i + r] + roff);
+        } else {
+          bd.win_rows.push_back(-1);
+        }
+      }
+      bd.win_cols.emplace_back();
+    }
+  }
+  (void)base_w;
+  for (int32_t r = 0; r < M; ++r) {
+    bd.row_win.push_back(local_row_win[r]);
+    bd.row_slot.push_back(local_row_slot[r]);
+  }
+
+  // 3. dense columns per window (merged across tiles and groups).
+  for (int gi = 0; gi < pk->num_tc; ++gi) {
+    const StileTcGroup& g = pk->tc[gi];
+    const int Kg = g.group_size;
+    for (int s = 0; s < g.num_windows; ++s) {
+      const int32_t win = group_slot_win[gi][s];
+      for (int32_t t = g.indptr[s]; t < g.indptr[s + 1]; ++t) {
+        for (int kk = 0; kk < Kg; ++kk) {
+          int32_t col = g.k_notsorted ? g.indices[t] * 16 + kk : g.indices[(int64_t)t * Kg + kk];
+          DenseCol dc;
+          dc.col = col + coff;
+          bool any = false;
+          for (int r = 0; r < 16; ++r) {
+            float v = g.a[((int64_t)t * 16 + r) * Kg + kk];
+            dc.v[r] = v;
+            if (v != 0.f) {
+              if (win < 0) return fail(STILE_E_INVALID, "TC group: nonzero value in a window without rows");
+              if (bd.win_rows[(size_t)win * 16 + r] < 0)
+                return fail(STILE_E_INVALID, "TC group: nonzero value in a padding row");
+              any = true;
+              bd.block_nnz++;
+            }
+          }
+          if (!any) continue;
+          if (col < 0 || col >= K) return fail(STILE_E_INVALID, "TC group: column index out of range");
+          bd.win_cols[win].push_back(dc);
+        }
+      }
+    }
+  }
+
+  // 4. sparse (ELL) products.
+  for (int gi = 0; gi < pk->num_ell; ++gi) {
+    const StileEllGroup& g = pk->ell[gi];
+    if (g.m_blk <= 0 || g.nnz_cols <= 0 || g.num_tiles < 0)
+      return fail(STILE_E_INVALID, "ELL group: bad m_blk/nnz_cols/num_tiles");
+    if (g.num_tiles > 0 && (!g.a || !g.indices_k || !g.indices_reordered_i))
+      return fail(STILE_E_INVALID, "ELL group: null buffer");
+    const int I = g.m_blk, Kc = g.nnz_cols;
+    for (int64_t t = 0; t < g.num_tiles; ++t) {
+      for (int i = 0; i < I; ++i) {
+        const int64_t tr = t * I + i;
+        const int32_t row = g.indices_reordered_i[tr];
+        if (row < 0 || row > M) return fail(STILE_E_INVALID, "ELL group: row index out of range");
+        for (int kk = 0; kk < Kc; ++kk) {
+          const float v = g.a[t * (int64_t)I * Kc + (int64_t)i * Kc + kk];
+          if (v == 0.f) continue;   // padding slots and explicit zeros add nothing
+          if (row == M) continue;   // sentinel row: products are discarded
+          const int32_t col = g.indices_k[tr * Kc + kk];
+          if (col < 0 || col >= K) return fail(STILE_E_INVALID, "ELL group: column index out of range");
+          bd.s_row.push_back(row + roff);
+          bd.s_col.push_back(col + coff);
+          bd.s_val.push_back(v);
+        }
+      }
+    }
+  }
+  (void)base_w;
+  return STILE_OK;
+}
+
+int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
+  const int32_t W = (int32_t)(bd.win_rows.size() / 16);
+  const int64_t Mtot = (int64_t)bd.row_win.size();
+  // Global row -> (window, slot): row_win holds pack-local windows already
+  // offset (windows are appended in pack order), see add_spmm_pack.
+  // Sparse part as CSR over window slots, columns ascending within a slot.
+  std::vector<int64_t> cnt((size_t)W * 16 + 1, 0);
+  std::vector<int32_t> slot_of(bd.s_row.size());
+  for (size_t e = 0; e < bd.s_row.size(); ++e) {
+    const int32_t r = bd.s_row[e];
+    const int32_t slot = bd.row_win[r] * 16 + bd.row_slot[r];
+    slot_of[e] = slot;
+    cnt[slot + 1]++;
+  }
+  for (size_t i = 1; i < cnt.size(); ++i) cnt[i] += cnt[i - 1];
+  const int64_t nres = cnt.back();
+  if (nres > INT32_MAX) return fail(STILE_E_UNSUPPORTED, "more than 2^31 sparse nonzeros");
+  std::vector<int32_t> res_ptr(cnt.begin(), cnt.end());
+  std::vector<int32_t> res_cols(nres);
+  std::vector<float> res_vals(nres);
+  {
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    for (size_t e = 0; e < bd.s_row.size(); ++e) {
+      const int64_t at = fill[slot_of[e]]++;
+      res_cols[at] = bd.s_col[e];
+      res_vals[at] = bd.s_val[e];
+    }
+    std::vector<std::pair<int32_t, float>> tmp;
+    for (size_t sl = 0; sl + 1 < cnt.size(); ++sl) {
+      const int64_t a = cnt[sl], b = cnt[sl + 1];
+      if (b - a < 2) continue;
+      tmp.resize(b - a);
+      for (int64_t q = a; q < b; ++q) tmp[q - a] = {res_cols[q], res_vals[q]};
+      std::stable_sort(tmp.begin(), tmp.end(),
+                       [](const std::pair<int32_t, float>& x, const std::pair<int32_t, float>& y) {
+                         return x.first < y.first;
+                       });
+      for (int64_t q = a; q < b; ++q) {
+        res_cols[q] = tmp[q - a].first;
+        res_vals[q] = tmp[q - a].second;
+      }
+    }
+  }
+  int32_t hub_rows = 0;
+  for (size_t sl = 0; sl + 1 < cnt.size(); ++sl)
+    if (cnt[sl + 1] - cnt[sl] > stile::kHubNnz) hub_rows++;
+
+  // Dense chunks: merge duplicate columns of a window, sort by column, cut
+  // into chunks of 32 k-slots (the last one padded with value-0 slots that
+  // point at the chunk's first column).
+  std::vector<int32_t> tc_ptr(W + 1, 0);
+  int64_t nchunks = 0;
+  for (int32_t w = 0; w < W; ++w) {
+    auto& cols = bd.win_cols[w];
+    std::sort(cols.begin(), cols.end(), [](const DenseCol& a, const DenseCol& b) { return a.col < b.col; });
+    size_t o = 0;
+    for (size_t i = 0; i < cols.size(); ++i) {
+      if (o > 0 && cols[o - 1].col == cols[i].col) {
+        for (int r = 0; r < 16; ++r) cols[o - 1].v[r] += cols[i].v[r];
+      } else {
+        cols[o++] = cols[i];
+      }
+    }
+    cols.resize(o);
+    nchunks += (int64_t)(cols.size() + 31) / 32;
+    tc_ptr[w + 1] = (int32_t)nchunks;
+  }
+  if (nchunks > INT32_MAX / 64) return fail(STILE_E_UNSUPPORTED, "too many dense chunks");
+  const size_t val_bytes = blk_dtype == STILE_BLK_F32 ? 4 : 2;
+  std::vector<int32_t> tc_cols((size_t)nchunks * 32);
+  std::vector<unsigned char> tc_vals((size_t)nchunks * 64 * 8 * val_bytes);
+  std::vector<float> tc_scale(blk_dtype == STILE_BLK_F16 ? nchunks : 0);
+  for (int32_t w = 0; w < W; ++w) {
+    const auto& cols = bd.win_cols[w];
+    for (int64_t c = tc_ptr[w]; c < tc_ptr[w + 1]; ++c) {
+      const size_t base = (size_t)(c - tc_ptr[w]) * 32;
+      float scale = 1.f;
+      if (blk_dtype == STILE_BLK_F16) {
+        float amax = 0.f;
+        for (size_t ks = 0; ks < 32 && base + ks < cols.size(); ++ks)
+          for (int r = 0; r < 16; ++r) amax = std::max(amax, std::fabs(cols[base + ks].v[r]));
+        // Scale the chunk so its largest magnitude maps to 2^14 (well inside
+        // FP16's range, far above its subnormals).
+        if (amax > 0.f) scale = std::ldexp(1.f, std::ilogb(amax) - 14);
+        tc_scale[c] = scale;
+      }
+      for (int g = 0; g < 4; ++g)
+        for (int e = 0; e < 8; ++e) {
+          const size_t ks = 4 * e + g;
+          tc_cols[(size_t)c * 32 + g * 8 + e] =
+              base + ks < cols.size() ? cols[base + ks].col : cols[base].col;
+        }
+      for (int lane = 0; lane < 64; ++lane) {
+        const int row = lane & 15, g = lane >> 4;
+        for (int e = 0; e < 8; ++e) {
+          const size_t ks = 4 * e + g;
+          const float v = base + ks < cols.size() ? cols[base + ks].v[row] : 0.f;
+          const size_t at = ((size_t)c * 64 + lane) * 8 + e;
+          if (blk_dtype == STILE_BLK_F32) {
+            std::memcpy(&tc_vals[at * 4], &v, 4);
+          } else if (blk_dtype == STILE_BLK_BF16) {
+            uint16_t b = to_bf16_bits(v);
+            std::memcpy(&tc_vals[at * 2], &b, 2);
+          } else {
+            uint16_t b = to_f16_bits(v / scale);
+            std::memcpy(&tc_vals[at * 2], &b, 2);
+          }
+        }
+      }
+    }
+  }
+
+  Layout lay;
+  const size_t o_rows = lay.add(bd.win_rows.size() * 4);
+  const size_t o_tcptr = lay.add(tc_ptr.size() * 4);
+  const size_t o_tccols = lay.add(tc_cols.size() * 4);
+  const size_t o_tcvals = lay.add(tc_vals.size());
+  const size_t o_scale = lay.add(tc_scale.size() * 4);
+  const size_t o_resptr = lay.add(res_ptr.size() * 4);
+  const size_t o_rescols = lay.add(res_cols.size() * 4);
+  const size_t o_resvals = lay.add(res_vals.size() * 4);
+  HostBlob blob;
+  blob.data.assign(lay.bytes, 0);
+  blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
+  blob.put(o_tcptr, tc_ptr.data(), tc_ptr.size() * 4);
+  blob.put(o_tccols, tc_cols.data(), tc_cols.size() * 4);
+  blob.put(o_tcvals, tc_vals.data(), tc_vals.size());
+  blob.put(o_scale, tc_scale.data(), tc_scale.size() * 4);
+  blob.put(o_resptr, res_ptr.data(), res_ptr.size() * 4);
+  blob.put(o_rescols, res_cols.data(), res_cols.size() * 4);
+  blob.put(o_resvals, res_vals.data(), res_vals.size() * 4);
+
+  StilePlan* plan = new (std::nothrow) StilePlan();
+  if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
+  plan->kind = 1;
+  plan->blk_dtype = blk_dtype;
+  int rc = upload(plan, lay, blob);
+  if (rc != STILE_OK) {
+    if (plan->dev) (void)hipFree(plan->dev);
+    delete plan;
+    return rc;
+  }
+  stile::SpmmPlanDev& d = plan->spmm;
+  d.M = bd.M;
+  d.K = bd.K;
+  d.W = W;
+  d.nchunks = (int32_t)nchunks;
+  d.blk_dtype = blk_dtype;
+  d.win_rows = dptr<int32_t>(plan, o_rows);
+  d.tc_ptr = dptr<int32_t>(plan, o_tcptr);
+  d.tc_cols = dptr<int32_t>(plan, o_tccols);
+  d.tc_vals = dptr<unsigned char>(plan, o_tcvals);
+  d.tc_scale = tc_scale.empty() ? nullptr : dptr<float>(plan, o_scale);
+  d.res_ptr = dptr<int32_t>(plan, o_resptr);
+  d.res_cols = dptr<int32_t>(plan, o_rescols);
+  d.res_vals = dptr<float>(plan, o_resvals);
+  StilePlanInfo& in = plan->info;
+  in.kind = 1;
+  in.blk_dtype = blk_dtype;
+  in.M = bd.M;
+  in.K = bd.K;
+  in.num_windows = W;
+  in.num_chunks = (int32_t)nchunks;
+  in.block_nnz = bd.block_nnz;
+  in.sparse_nnz = nres;
+  in.device_bytes = (int64_t)plan->dev_bytes;
+  in.hub_rows = hub_rows;
+  in.num_split = 0;
+  (void)Mtot;
+  *out = plan;
+  return STILE_OK;
+}
+
+int create_spmm(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype, StilePlan** out) {
+  if (!out) return fail(STILE_E_INVALID, "null output pointer");
+  *out = nullptr;
+  if (!packs || count <= 0) return fail(STILE_E_INVALID, "no packs");
+  if (blk_dtype < STILE_BLK_F32 || blk_dtype > STILE_BLK_F16)
+    return fail(STILE_E_INVALID, "unknown blk_dtype");
+  SpmmBuild bd;
+  int64_t roff = 0, coff = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    const size_t w0 = bd.win_rows.size() / 16;
+    const size_t r0 = bd.row_win.size();
+    int rc = add_spmm_pack(bd, &packs[i], (int32_t)roff, (int32_t)coff);
+    if (rc != STILE_OK) return rc;
+    (void)w0;
+    (void)r0;
+    roff += packs[i].M;
+    coff += packs[i].K;
+    if (roff > INT32_MAX || coff > INT32_MAX) return fail(STILE_E_UNSUPPORTED, "batched size exceeds 2^31");
+  }
+  bd.M = (int32_t)roff;
+  bd.K = (int32_t)coff;
+  return finish_spmm(bd, blk_dtype, out);
+}
+
+// ---------------------------------------------------------------------------
+// SDDMM plan construction.
+// ---------------------------------------------------------------------------
+struct SddmmBuild {
+  int32_t M = 0, N = 0, nnz = 0;
+  std::vector<int32_t> e_row, e_col, e_out;
+  std::vector<int32_t> b_rows, b_cols, b_ptr{0}, b_pos, b_out;
+};
+
+int add_sddmm_pack(SddmmBuild& bd, const StileSddmmPack* pk, int32_t roff, int32_t coff, int32_t ooff) {
+  if (!pk) return fail(STILE_E_INVALID, "null pack");
+  if (pk->M < 0 || pk->N < 0 || pk->nnz < 0) return fail(STILE_E_INVALID, "negative size");
+  if ((pk->num_1d > 0 && !pk->g1d) || (pk->num_tc > 0 && !pk->tc) || pk->num_1d < 0 || pk->num_tc < 0)
+    return fail(STILE_E_INVALID, "group array missing");
+  std::vector<unsigned char> seen(pk->nnz, 0);
+  for (int gi = 0; gi < pk->num_1d; ++gi) {
+    const StileSddmm1DGroup& g = pk->g1d[gi];
+    if (g.len < 0 || (g.len > 0 && (!g.mid_idx || !g.j_idx || !g.remap)))
+      return fail(STILE_E_INVALID, "1D group: bad buffers");
+    int32_t len = g.len;
+    // Trailing padding repeats the last entry (my_fuse_formats.py:3586-3588).
+    while (len > 1 && g.mid_idx[len - 1] == g.mid_idx[len - 2] && g.j_idx[len - 1] == g.j_idx[len - 2] &&
+           g.remap[len - 1] == g.remap[len - 2])
+      --len;
+    for (int32_t e = 0; e < len; ++e) {
+      const int32_t r = g.mid_idx[e], c = g.j_idx[e], o = g.remap[e];
+      if (r < 0 || r >= pk->M || c < 0 || c >= pk->N || o < 0 || o >= pk->nnz)
+        return fail(STILE_E_INVALID, "1D group: index out of range");
+      seen[o] = 1;
+      bd.e_row.push_back(r + roff);
+      bd.e_col.push_back(c + coff);
+      bd.e_out.push_back(o + ooff);
+    }
+  }
+  for (int gi = 0; gi < pk->num_tc; ++gi) {
+    const StileSddmmTcGroup& g = pk->tc[gi];
+    if (g.nnzb < 0) return fail(STILE_E_INVALID, "TC group: negative nnzb");
+    if (g.blk_k != 16) return fail(STILE_E_UNSUPPORTED, "TC SDDMM blocks must be 16 columns wide");
+    if (g.nnzb > 0 && (!g.i_indices || !g.j_indices || !g.indptr || !g.indices_dense || !g.indices))
+      return fail(STILE_E_INVALID, "TC group: null buffer");
+    if (g.nnzb > 0 && g.indptr[0] != 0) return fail(STILE_E_INVALID, "TC group: indptr[0] != 0");
+    for (int32_t b = 0; b < g.nnzb; ++b) {
+      for (int r = 0; r < 16; ++r) {
+        const int32_t ri = g.i_indices[(int64_t)b * 16 + r], ci = g.j_indices[(int64_t)b * 16 + r];
+        if (ri < 0 || ri >= pk->M || ci < 0 || ci >= pk->N)
+          return fail(STILE_E_INVALID, "TC group: block row/col out of range");
+        bd.b_rows.push_back(ri + roff);
+        bd.b_cols.push_back(ci + coff);
+      }
+      const int32_t s0 = g.indptr[b], s1 = g.indptr[b + 1];
+      if (s1 < s0) return fail(STILE_E_INVALID, "TC group: indptr not monotone");
+      for (int32_t s = s0; s < s1; ++s) {
+        const int32_t pos = g.indices_dense[s], o = g.indices[s];
+        if (pos < 0 || pos >= 256 || o < 0 || o >= pk->nnz)
+          return fail(STILE_E_INVALID, "TC group: sampled entry out of range");
+        seen[o] = 1;
+        bd.b_pos.push_back(pos);
+        bd.b_out.push_back(o + ooff);
+      }
+      bd.b_ptr.push_back((int32_t)bd.b_pos.size());
+    }
+  }
+  for (int32_t o = 0; o < pk->nnz; ++o)
+    if (!seen[o]) return fail(STILE_E_INVALID, "packed groups leave an output entry uncovered");
+  return STILE_OK;
+}
+
+int create_sddmm(const StileSddmmPack* packs, int32_t count, int32_t blk_dtype, StilePlan** out) {
+  if (!out) return fail(STILE_E_INVALID, "null output pointer");
+  *out = nullptr;
+  if (!packs || count <= 0) return fail(STILE_E_INVALID, "no packs");
+  if (blk_dtype != STILE_BLK_F32)
+    return fail(STILE_E_UNSUPPORTED, "SDDMM blocks run FP32 MFMA only");
+  SddmmBuild bd;
+  int64_t roff = 0, coff = 0, ooff = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    int rc = add_sddmm_pack(bd, &packs[i], (int32_t)roff, (int32_t)coff, (int32_t)ooff);
+    if (rc != STILE_OK) return rc;
+    roff += packs[i].M;
+    coff += packs[i].N;
+    ooff += packs[i].nnz;
+    if (roff > INT32_MAX || coff > INT32_MAX || ooff > INT32_MAX)
+      return fail(STILE_E_UNSUPPORTED, "batched size exceeds 2^31");
+  }
+  bd.M = (int32_t)roff;
+  bd.N = (int32_t)coff;
+  bd.nnz = (int32_t)ooff;
+  Layout lay;
+  const size_t o_er = lay.add(bd.e_row.size() * 4), o_ec = lay.add(bd.e_col.size() * 4),
+               o_eo = lay.add(bd.e_out.size() * 4), o_br = lay.add(bd.b_rows.size() * 4),
+               o_bc = lay.add(bd.b_cols.size() * 4), o_bp = lay.add(bd.b_ptr.size() * 4),
+               o_bs = lay.add(bd.b_pos.size() * 4), o_bo = lay.add(bd.b_out.size() * 4);
+  HostBlob blob;
+  blob.data.assign(lay.bytes, 0);
+  blob.put(o_er, bd.e_row.data(), bd.e_row.size() * 4);
+  blob.put(o_ec, bd.e_col.data(), bd.e_col.size() * 4);
+  blob.put(o_eo, bd.e_out.data(), bd.e_out.size() * 4);
+  blob.put(o_br, bd.b_rows.data(), bd.b_rows.size() * 4);
+  blob.put(o_bc, bd.b_cols.data(), bd.b_cols.size() * 4);
+  blob.put(o_bp, bd.b_ptr.data(), bd.b_ptr.size() * 4);
+  blob.put(o_bs, bd.b_pos.data(), bd.b_pos.size() * 4);
+  blob.put(o_bo, bd.b_out.data(), bd.b_out.size() * 4);
+  StilePlan* plan = new (std::nothrow) StilePlan();
+  if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
+  plan->kind = 2;
+  plan->blk_dtype = blk_dtype;
+  int rc = upload(plan, lay, blob);
+  if (rc != STILE_OK) {
+    if (plan->dev) (void)hipFree(plan->dev);
+    delete plan;
+    return rc;
+  }
+  stile::SddmmPlanDev& d = plan->sddmm;
+  d.M = bd.M;
+  d.N = bd.N;
+  d.nnz = bd.nnz;
+  d.n1d = (int32_t)bd.e_row.size();
+  d.nblk = (int32_t)(bd.b_ptr.size() - 1);
+  d.blk_dtype = blk_dtype;
+  d.e_row = dptr<int32_t>(plan, o_er);
+  d.e_col = dptr<int32_t>(plan, o_ec);
+  d.e_out = dptr<int32_t>(plan, o_eo);
+  d.blk_rows = dptr<int32_t>(plan, o_br);
+  d.blk_cols = dptr<int32_t>(plan, o_bc);
+  d.blk_ptr = dptr<int32_t>(plan, o_bp);
+  d.blk_pos = dptr<int32_t>(plan, o_bs);
+  d.blk_out = dptr<int32_t>(plan, o_bo);
+  StilePlanInfo& in = plan->info;
+  in.kind = 2;
+  in.blk_dtype = blk_dtype;
+  in.M = bd.M;
+  in.K = bd.N;
+  in.num_windows = 0;
+  in.num_chunks = d.nblk;
+  in.block_nnz = (int64_t)bd.b_pos.size();
+  in.sparse_nnz = (int64_t)bd.e_row.size();
+  in.device_bytes = (int64_t)plan->dev_bytes;
+  in.hub_rows = 0;
+  in.num_split = 0;
+  *out = plan;
+  return STILE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t stile_abi_version(void) { return 1; }
+
+const char* stile_last_error(void) { return g_err.c_str(); }
+
+int stile_csr_spmm_f32(int32_t M, int32_t K, int32_t N, const int32_t* indptr, const int32_t* indices,
+                       const float* vals, const float* B, int64_t ldb, float* C, int64_t ldc, void* stream) {
+  if (M < 0 || K < 0 || N < 0) return fail(STILE_E_INVALID, "negative size");
+  if (M == 0 || N == 0) return STILE_OK;
+  if (!indptr || !C || (K > 0 && !B)) return fail(STILE_E_INVALID, "null pointer");
+  if (ldb < N || ldc < N) return fail(STILE_E_INVALID, "leading dimension smaller than N");
+  if (K == 0) {
+    hipError_t e = stile::launch_fill_zero_2d(C, M, N, ldc, (hipStream_t)stream);
+    return e == hipSuccess ? STILE_OK : hip_fail(e, "fill_zero launch");
+  }
+  hipError_t e = stile::launch_csr_spmm_f32(M, N, indptr, indices, vals, B, ldb, C, ldc, (hipStream_t)stream);
+  return e == hipSuccess ? STILE_OK : hip_fail(e, "csr_spmm launch");
+}
+
+int stile_csr_sddmm_f32(int32_t M, int32_t N, int32_t nnz, int32_t Kf, const int32_t* indptr,
+                        const int32_t* indices, const float* X, int64_t ldx, const float* Y, int64_t ldy,
+                        float* out, void* stream) {
+  (void)N;
+  if (M < 0 || N < 0 || nnz < 0 || Kf < 0) return fail(STILE_E_INVALID, "negative size");
+  if (nnz == 0) return STILE_OK;
+  if (!indptr || !indices || !out) return fail(STILE_E_INVALID, "null pointer");
+  if (Kf == 0) {
+    hipError_t e = stile::launch_fill_zero(out, nnz, (hipStream_t)stream);
+    return e == hipSuccess ? STILE_OK : hip_fail(e, "fill_zero launch");
+  }
+  if (!X || !Y) return fail(STILE_E_INVALID, "null pointer");
+  if (ldx < Kf || ldy < Kf) return fail(STILE_E_INVALID, "leading dimension smaller than Kf");
+  hipError_t e = stile::launch_csr_sddmm_f32(M, nnz, Kf, indptr, indices, X, ldx, Y, ldy, out, (hipStream_t)stream);
+  return e == hipSuccess ? STILE_OK : hip_fail(e, "csr_sddmm launch");
+}
+
+int stile_plan_create_spmm(const StileSpmmPack* pack, int32_t blk_dtype, StilePlan** out) {
+  return create_spmm(pack, 1, blk_dtype, out);
+}
+
+int stile_plan_create_spmm_batched(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype,
+                                   StilePlan** out) {
+  return create_spmm(packs, count, blk_dtype, out);
+}
+
+int stile_plan_create_sddmm(const StileSddmmPack* pack, int32_t blk_dtype, StilePlan** out) {
+  return create_sddmm(pack, 1, blk_dtype, out);
+}
+
+int stile_plan_create_sddmm_batched(const StileSddmmPack* packs, int32_t count, int32_t blk_dtype,
+                                    StilePlan** out) {
+  return create_sddmm(packs, count, blk_dtype, out);
+}
+
+int stile_spmm_run(const StilePlan* plan, const float* B, int64_t ldb, int32_t N, float* C, int64_t ldc,
+                   void* stream) {
+  if (!plan || plan->kind != 1) return fail(STILE_E_INVALID, "not an SpMM plan");
+  if (N < 0) return fail(STILE_E_INVALID, "negative N");
+  if (N == 0 || plan->spmm.W == 0) return STILE_OK;
+  if (!C || (plan->spmm.K > 0 && !B)) return fail(STILE_E_INVALID, "null pointer");
+  if (ldb < N || ldc < N) return fail(STILE_E_INVALID, "leading dimension smaller than N");
+  hipError_t e = stile::launch_hyb_spmm(plan->spmm, B, ldb, N, C, ldc, (hipStream_t)stream);
+  return e == hipSuccess ? STILE_OK : hip_fail(e, "hyb_spmm launch");
+}
+
+int stile_sddmm_run(const StilePlan* plan, const float* X, int64_t ldx, const float* Y, int64_t ldy, int32_t Kf,
+                    float* out, void* stream) {
+  if (!plan || plan->kind != 2) return fail(STILE_E_INVALID, "not an SDDMM plan");
+  if (Kf < 0) return fail(STILE_E_INVALID, "negative Kf");
+  if (plan->sddmm.nnz == 0) return STILE_OK;
+  if (!out || !X || !Y) return fail(STILE_E_INVALID, "null pointer");
+  if (ldx < Kf || ldy < Kf) return fail(STILE_E_INVALID, "leading dimension smaller than Kf");
+  if (Kf == 0) {
+    hipError_t e = stile::launch_fill_zero(out, plan->sddmm.nnz, (hipStream_t)stream);
+    return e == hipSuccess ? STILE_OK : hip_fail(e, "fill_zero launch");
+  }
+  hipError_t e = stile::launch_hyb_sddmm(plan->sddmm, X, ldx, Y, ldy, Kf, out, (hipStream_t)stream);
+  return e == hipSuccess ? STILE_OK : hip_fail(e, "hyb_sddmm launch");
+}
+
+int stile_plan_info(const StilePlan* plan, StilePlanInfo* info) {
+  if (!plan || !info) return fail(STILE_E_INVALID, "null pointer");
+  *info = plan->info;
+  return STILE_OK;
+}
+
+int stile_plan_destroy(StilePlan* plan) {
+  if (!plan) return STILE_OK;
+  int rc = STILE_OK;
+  if (plan->dev) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (cur != plan->device) (void)hipSetDevice(plan->device);
+    hipError_t e = hipFree(plan->dev);
+    if (cur != plan->device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) rc = hip_fail(e, "hipFree of plan");
+  }
+  delete plan;
+  return rc;
+}
+
+}  // extern "C"
