@@ -11,6 +11,11 @@ namespace stile {
 // nonzeros is summed with compensated (Neumaier) FP32 arithmetic.
 constexpr int kHubNnz = 4096;
 
+// Rows whose sparse part is longer than this are cut: the window pass sums
+// the first kSplitNnz nonzeros, the rest go to pieces of kSplitNnz that a
+// fix-up kernel adds with atomicAdd after the window pass.
+constexpr int kSplitNnz = 256;
+
 // Device view of a hybrid SpMM plan (window-major execution layout).
 //
 // The rows of C are partitioned into 16-row windows (the reference's TC row
@@ -26,8 +31,13 @@ constexpr int kHubNnz = 4096;
 //   res_ptr[16w + r] .. res_ptr[16w + r + 1] -> (res_cols, res_vals).
 struct SpmmPlanDev {
   int32_t M, K, W;
+  int32_t W_tc;              // windows [0, W_tc) hold dense chunks; the rest are sparse-only
   int32_t nchunks;
   int32_t blk_dtype;
+  int32_t npieces;           // split pieces of long sparse rows
+  const int32_t* piece_row;  // [npieces] output row
+  const int32_t* piece_beg;  // [npieces] range in res_cols/res_vals
+  const int32_t* piece_end;
   const int32_t* win_rows;   // [W*16]
   const int32_t* tc_ptr;     // [W+1]
   const int32_t* tc_cols;    // [nchunks*32]
@@ -66,6 +76,8 @@ hipError_t launch_csr_sddmm_f32(int M, int nnz, int Kf, const int32_t* indptr,
                                 const int32_t* indices, const float* X,
                                 int64_t ldx, const float* Y, int64_t ldy,
                                 float* out, hipStream_t s);
+// One-time kernel attributes (dynamic LDS above 64 KiB) for a plan.
+void prepare_spmm_kernels(const SpmmPlanDev& p);
 hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb,
                            int N, float* C, int64_t ldc, hipStream_t s);
 hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,

@@ -18,6 +18,10 @@
 
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
 namespace stile {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -293,154 +297,507 @@ hipError_t launch_csr_sddmm_f32(int M, int nnz, int Kf, const int32_t* indptr,
 //         is k-slot 4e + g for both A and B; the pairing is what matters).
 //   FP32: eight v_mfma_f32_16x16x4_f32 per tile (MFMA e reduces k-slots
 //         4e + 0..3; lane group g supplies k-slot 4e + g).
-// Sparse part: the lane adds v * B[col][n0+4j..+3] for every FP32 nonzero of
-// its 4 rows (compensated when a row has more than kHubNnz of them).
+// Sparse part: the lane adds v * B[col][n0+4j..+3] for the first kSplitNnz
+// FP32 nonzeros of its rows (compensated when a row has more than kHubNnz);
+// the rest of a long row is added by hyb_spmm_fixup_kernel.
+//
+// Windows [0, W_tc) (with dense chunks) take one wave per 64 columns and a
+// lane group runs 4 rows' sparse parts; sparse-only windows take four waves
+// per 64 columns and a lane group runs one row.
 // ---------------------------------------------------------------------------
+
+// Sum of v * B[col][n0+4j..+3] over nonzeros [r0, r1) of the sparse part.
+__device__ __forceinline__ float4 sparse_row_sum(const SpmmPlanDev& p, const float* __restrict__ B,
+                                                 int64_t ldb, int N, int col, bool vec, int r0, int r1,
+                                                 bool compensated) {
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (!compensated) {
+    int q = r0;
+    for (; q + 4 <= r1; q += 4) {
+      const int k0 = p.res_cols[q], k1 = p.res_cols[q + 1], k2 = p.res_cols[q + 2], k3 = p.res_cols[q + 3];
+      const float v0 = p.res_vals[q], v1 = p.res_vals[q + 1], v2 = p.res_vals[q + 2], v3 = p.res_vals[q + 3];
+      const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
+      const float4 b1 = load_b4(B, ldb, k1, col, N, vec);
+      const float4 b2 = load_b4(B, ldb, k2, col, N, vec);
+      const float4 b3 = load_b4(B, ldb, k3, col, N, vec);
+      s.x += v0 * b0.x; s.y += v0 * b0.y; s.z += v0 * b0.z; s.w += v0 * b0.w;
+      s.x += v1 * b1.x; s.y += v1 * b1.y; s.z += v1 * b1.z; s.w += v1 * b1.w;
+      s.x += v2 * b2.x; s.y += v2 * b2.y; s.z += v2 * b2.z; s.w += v2 * b2.w;
+      s.x += v3 * b3.x; s.y += v3 * b3.y; s.z += v3 * b3.z; s.w += v3 * b3.w;
+    }
+    for (; q < r1; ++q) {
+      const int k0 = p.res_cols[q];
+      const float v0 = p.res_vals[q];
+      const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
+      s.x += v0 * b0.x; s.y += v0 * b0.y; s.z += v0 * b0.z; s.w += v0 * b0.w;
+    }
+  } else {
+    float4 cc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int q = r0; q < r1; ++q) {
+      const int k0 = p.res_cols[q];
+      const float v0 = p.res_vals[q];
+      const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
+      neu_add(s.x, cc.x, v0 * b0.x);
+      neu_add(s.y, cc.y, v0 * b0.y);
+      neu_add(s.z, cc.z, v0 * b0.z);
+      neu_add(s.w, cc.w, v0 * b0.w);
+    }
+    s.x += cc.x; s.y += cc.y; s.z += cc.z; s.w += cc.w;
+  }
+  return s;
+}
+
+// Output rows and sparse ranges of the 4 slots a lane group finishes.  Loaded
+// before any of the window's C stores: vector-memory counters complete in
+// issue order on CDNA, so a load issued after a store would wait for it.
+struct SlotInfo {
+  int row[4];
+  int r0[4];
+  int rend[4];
+};
+
+__device__ __forceinline__ SlotInfo load_slots(const SpmmPlanDev& p, int slot0) {
+  SlotInfo si;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    si.row[i] = p.win_rows[slot0 + i];
+    si.r0[i] = p.res_ptr[slot0 + i];
+  }
+  si.rend[0] = si.r0[1];
+  si.rend[1] = si.r0[2];
+  si.rend[2] = si.r0[3];
+  si.rend[3] = p.res_ptr[slot0 + 4];
+  return si;
+}
+
+// Add the sparse parts of the 4 slots to the dense accumulators, then store
+// the 4 rows (all loads of the window happen before its first store).
+__device__ __forceinline__ void finish_window(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
+                                              int N, float* __restrict__ C, int64_t ldc, int col, bool vec,
+                                              const SlotInfo& si, const f32x4 (&acc)[4]) {
+  float4 o[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    o[i] = make_float4(acc[0][i], acc[1][i], acc[2][i], acc[3][i]);
+    const int r0 = si.r0[i], rend = si.rend[i];
+    if (rend > r0) {
+      const int r1 = min(rend, r0 + kSplitNnz);
+      const float4 sp = sparse_row_sum(p, B, ldb, N, col, vec, r0, r1, rend - r0 > kHubNnz);
+      o[i].x += sp.x;
+      o[i].y += sp.y;
+      o[i].z += sp.z;
+      o[i].w += sp.w;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (si.row[i] >= 0) store_c4(C, ldc, si.row[i], col, N, vec, o[i]);
+}
+
+// One slot (sparse-only windows, one row per lane group).
+__device__ __forceinline__ void finish_slot(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
+                                            int N, float* __restrict__ C, int64_t ldc, int col, bool vec,
+                                            int slot, float4 dense) {
+  const int row = p.win_rows[slot];
+  const int r0 = p.res_ptr[slot], rend = p.res_ptr[slot + 1];
+  const int r1 = min(rend, r0 + kSplitNnz);
+  float4 s = sparse_row_sum(p, B, ldb, N, col, vec, r0, r1, rend - r0 > kHubNnz);
+  if (row >= 0) {
+    float4 o;
+    o.x = dense.x + s.x;
+    o.y = dense.y + s.y;
+    o.z = dense.z + s.z;
+    o.w = dense.w + s.w;
+    store_c4(C, ldc, row, col, N, vec, o);
+  }
+}
+
 template <int MODE>
-__device__ __forceinline__ void hyb_spmm_window(const SpmmPlanDev& p,
-                                                const float* __restrict__ B,
-                                                int64_t ldb, int N,
-                                                float* __restrict__ C,
-                                                int64_t ldc, int w, int n0,
+__device__ __forceinline__ void mfma_chunk(const SpmmPlanDev& p, int64_t c, int lane, const float4 (&b)[8],
+                                           f32x4 (&acc)[4]) {
+  if constexpr (MODE == 1) {
+    const bf16x8 a = reinterpret_cast<const bf16x8*>(p.tc_vals)[c * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      bf16x8 bt;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bt[e] = (__bf16)comp(b[e], t);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bt, acc[t], 0, 0, 0);
+    }
+  } else if constexpr (MODE == 2) {
+    const f16x8 a = reinterpret_cast<const f16x8*>(p.tc_vals)[c * 64 + lane];
+    const float sc = p.tc_scale[c];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      f16x8 bt;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bt[e] = (_Float16)comp(b[e], t);
+      f32x4 tmp = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bt, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      acc[t] += sc * tmp;
+    }
+  } else {
+    const float* ap = reinterpret_cast<const float*>(p.tc_vals) + (c * 64 + lane) * 8;
+    const float4 a0 = ld4(ap), a1 = ld4(ap + 4);
+    const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], comp(b[e], t), acc[t], 0, 0, 0);
+    }
+  }
+}
+
+// B rows of one chunk for lane group g: 8 float4 loads (addresses from the
+// chunk's column ids, already in registers).
+__device__ __forceinline__ void load_chunk_b(const float* __restrict__ B, int64_t ldb, int col, int N, bool vec,
+                                             const int4& ka, const int4& kb, float4 (&b)[8]) {
+  b[0] = load_b4(B, ldb, ka.x, col, N, vec);
+  b[1] = load_b4(B, ldb, ka.y, col, N, vec);
+  b[2] = load_b4(B, ldb, ka.z, col, N, vec);
+  b[3] = load_b4(B, ldb, ka.w, col, N, vec);
+  b[4] = load_b4(B, ldb, kb.x, col, N, vec);
+  b[5] = load_b4(B, ldb, kb.y, col, N, vec);
+  b[6] = load_b4(B, ldb, kb.z, col, N, vec);
+  b[7] = load_b4(B, ldb, kb.w, col, N, vec);
+}
+
+// Dense part of window w for this lane's columns.  Two chunks per step (16
+// independent B loads in flight) and the column ids of the next two chunks
+// fetched before the current step's B loads, so a step waits for one memory
+// latency instead of two.
+template <int MODE>
+__device__ __forceinline__ void window_dense(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
+                                             int N, int col, bool vec, int w, int lane, f32x4 (&acc)[4]) {
+  const int g = lane >> 4;
+  const int c0 = p.tc_ptr[w], c1 = p.tc_ptr[w + 1];
+  if (c0 >= c1) return;
+  auto cols_of = [&](int c, int4& ka, int4& kb) {
+    const int4* cp = reinterpret_cast<const int4*>(p.tc_cols + (int64_t)c * 32 + g * 8);
+    ka = cp[0];
+    kb = cp[1];
+  };
+  int4 ka0, kb0, ka1, kb1;
+  cols_of(c0, ka0, kb0);
+  if (c0 + 1 < c1) cols_of(c0 + 1, ka1, kb1); else { ka1 = ka0; kb1 = kb0; }
+  int c = c0;
+  for (; c + 1 < c1; c += 2) {
+    int4 na0 = ka0, nb0 = kb0, na1 = ka1, nb1 = kb1;
+    if (c + 2 < c1) cols_of(c + 2, na0, nb0);
+    if (c + 3 < c1) cols_of(c + 3, na1, nb1);
+    float4 b0[8], b1[8];
+    load_chunk_b(B, ldb, col, N, vec, ka0, kb0, b0);
+    load_chunk_b(B, ldb, col, N, vec, ka1, kb1, b1);
+    mfma_chunk<MODE>(p, c, lane, b0, acc);
+    mfma_chunk<MODE>(p, c + 1, lane, b1, acc);
+    ka0 = na0; kb0 = nb0; ka1 = na1; kb1 = nb1;
+  }
+  if (c < c1) {
+    float4 b0[8];
+    load_chunk_b(B, ldb, col, N, vec, ka0, kb0, b0);
+    mfma_chunk<MODE>(p, c, lane, b0, acc);
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ void hyb_spmm_window(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
+                                                int N, float* __restrict__ C, int64_t ldc, int w, int n0,
                                                 bool vec) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
   const int j = lane & 15;
   const int col = n0 + 4 * j;
 
+  const SlotInfo si = load_slots(p, w * 16 + g * 4);
   f32x4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-  const int c0 = p.tc_ptr[w], c1 = p.tc_ptr[w + 1];
-  for (int c = c0; c < c1; ++c) {
-    const int4* cp = reinterpret_cast<const int4*>(p.tc_cols + (int64_t)c * 32 + g * 8);
-    const int4 ka = cp[0], kb = cp[1];
-    float4 b[8];
-    b[0] = load_b4(B, ldb, ka.x, col, N, vec);
-    b[1] = load_b4(B, ldb, ka.y, col, N, vec);
-    b[2] = load_b4(B, ldb, ka.z, col, N, vec);
-    b[3] = load_b4(B, ldb, ka.w, col, N, vec);
-    b[4] = load_b4(B, ldb, kb.x, col, N, vec);
-    b[5] = load_b4(B, ldb, kb.y, col, N, vec);
-    b[6] = load_b4(B, ldb, kb.z, col, N, vec);
-    b[7] = load_b4(B, ldb, kb.w, col, N, vec);
-    if constexpr (MODE == 1) {
-      const bf16x8 a = reinterpret_cast<const bf16x8*>(p.tc_vals)[(int64_t)c * 64 + lane];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        bf16x8 bt;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) bt[e] = (__bf16)comp(b[e], t);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bt, acc[t], 0, 0, 0);
-      }
-    } else if constexpr (MODE == 2) {
-      const f16x8 a = reinterpret_cast<const f16x8*>(p.tc_vals)[(int64_t)c * 64 + lane];
-      const float sc = p.tc_scale[c];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        f16x8 bt;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) bt[e] = (_Float16)comp(b[e], t);
-        f32x4 tmp = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-            a, bt, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-        acc[t] += sc * tmp;
-      }
-    } else {
-      const float* ap = reinterpret_cast<const float*>(p.tc_vals) + ((int64_t)c * 64 + lane) * 8;
-      const float4 a0 = ld4(ap), a1 = ld4(ap + 4);
-      const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], comp(b[e], t), acc[t], 0, 0, 0);
-      }
-    }
-  }
-
-  // FP32 sparse part + store.
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int slot = w * 16 + g * 4 + i;
-    const int r0 = p.res_ptr[slot], r1 = p.res_ptr[slot + 1];
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (r1 - r0 <= kHubNnz) {
-      int q = r0;
-      for (; q + 2 <= r1; q += 2) {
-        const int k0 = p.res_cols[q], k1 = p.res_cols[q + 1];
-        const float v0 = p.res_vals[q], v1 = p.res_vals[q + 1];
-        const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
-        const float4 b1 = load_b4(B, ldb, k1, col, N, vec);
-        s.x += v0 * b0.x; s.y += v0 * b0.y; s.z += v0 * b0.z; s.w += v0 * b0.w;
-        s.x += v1 * b1.x; s.y += v1 * b1.y; s.z += v1 * b1.z; s.w += v1 * b1.w;
-      }
-      if (q < r1) {
-        const int k0 = p.res_cols[q];
-        const float v0 = p.res_vals[q];
-        const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
-        s.x += v0 * b0.x; s.y += v0 * b0.y; s.z += v0 * b0.z; s.w += v0 * b0.w;
-      }
-    } else {
-      float4 cc = make_float4(0.f, 0.f, 0.f, 0.f);
-      for (int q = r0; q < r1; ++q) {
-        const int k0 = p.res_cols[q];
-        const float v0 = p.res_vals[q];
-        const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
-        neu_add(s.x, cc.x, v0 * b0.x);
-        neu_add(s.y, cc.y, v0 * b0.y);
-        neu_add(s.z, cc.z, v0 * b0.z);
-        neu_add(s.w, cc.w, v0 * b0.w);
-      }
-      s.x += cc.x; s.y += cc.y; s.z += cc.z; s.w += cc.w;
-    }
-    const int row = p.win_rows[slot];
-    if (row >= 0) {
-      float4 o;
-      o.x = acc[0][i] + s.x;
-      o.y = acc[1][i] + s.y;
-      o.z = acc[2][i] + s.z;
-      o.w = acc[3][i] + s.w;
-      store_c4(C, ldc, row, col, N, vec, o);
-    }
-  }
+  window_dense<MODE>(p, B, ldb, N, col, vec, w, lane, acc);
+  finish_window(p, B, ldb, N, C, ldc, col, vec, si, acc);
 }
 
+// Waves [wave_base, ...): first the dense windows (one wave per window and
+// 64 columns), then the sparse-only windows (four waves per window and 64
+// columns).
 template <int MODE>
-__global__ __launch_bounds__(256) void hyb_spmm_kernel(SpmmPlanDev p,
-                                                       const float* __restrict__ B,
-                                                       int64_t ldb, int N,
-                                                       float* __restrict__ C,
-                                                       int64_t ldc, int nC,
-                                                       bool vec) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int w = wave / nC;
+__global__ __launch_bounds__(256) void hyb_spmm_kernel(SpmmPlanDev p, const float* __restrict__ B,
+                                                       int64_t ldb, int N, float* __restrict__ C,
+                                                       int64_t ldc, int nC, int wave_base, bool vec) {
+  const int wave = wave_base + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int dense_waves = p.W_tc * nC;
+  if (wave < dense_waves) {
+    const int w = wave / nC;
+    hyb_spmm_window<MODE>(p, B, ldb, N, C, ldc, w, (wave - w * nC) * 64, vec);
+    return;
+  }
+  const int idx = wave - dense_waves;
+  const int w = p.W_tc + idx / (4 * nC);
   if (w >= p.W) return;
-  const int n0 = (wave - w * nC) * 64;
-  hyb_spmm_window<MODE>(p, B, ldb, N, C, ldc, w, n0, vec);
+  const int rest = idx - (w - p.W_tc) * 4 * nC;
+  const int quarter = rest & 3;
+  const int n0 = (rest >> 2) * 64;
+  const int lane = threadIdx.x & 63;
+  const int col = n0 + 4 * (lane & 15);
+  finish_slot(p, B, ldb, N, C, ldc, col, vec, w * 16 + quarter * 4 + (lane >> 4), make_float4(0.f, 0.f, 0.f, 0.f));
 }
 
-hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb,
-                           int N, float* C, int64_t ldc, hipStream_t s) {
+// B-stationary variant for BF16 plans with K <= kLdsMaxK: a workgroup stages
+// B[0:K, n0:n0+64] as BF16 in LDS (one 128-byte row per k, lane j's four
+// columns at byte 8j) and runs the dense windows [grp*wpg, (grp+1)*wpg) of
+// its column chunk against it, one window per wave at a time.  Each B row
+// then leaves L2 once per workgroup instead of once per window.
+constexpr int kLdsMaxK = 1024;
+
+__device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
+  const uint16_t a = __builtin_bit_cast(uint16_t, (__bf16)lo);
+  const uint16_t b = __builtin_bit_cast(uint16_t, (__bf16)hi);
+  return (uint32_t)a | ((uint32_t)b << 16);
+}
+
+__device__ __forceinline__ uint32_t half_sel(uint32_t x, uint32_t y, int hi) {
+  // (x.lo, y.lo) or (x.hi, y.hi) as one packed pair
+  return hi ? ((x >> 16) | (y & 0xffff0000u)) : ((x & 0xffffu) | (y << 16));
+}
+
+constexpr int kLdsThreads = 1024;
+
+__device__ __forceinline__ void lds_chunk_mfma(const uint2* __restrict__ bsh, int j, const int4& ka, const int4& kb,
+                                               const bf16x8& a, f32x4 (&acc)[4]) {
+  uint2 r[8];
+  r[0] = bsh[ka.x * 16 + j];
+  r[1] = bsh[ka.y * 16 + j];
+  r[2] = bsh[ka.z * 16 + j];
+  r[3] = bsh[ka.w * 16 + j];
+  r[4] = bsh[kb.x * 16 + j];
+  r[5] = bsh[kb.y * 16 + j];
+  r[6] = bsh[kb.z * 16 + j];
+  r[7] = bsh[kb.w * 16 + j];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    uint4 pk;
+    const int hi = t & 1;
+    if (t < 2) {
+      pk.x = half_sel(r[0].x, r[1].x, hi);
+      pk.y = half_sel(r[2].x, r[3].x, hi);
+      pk.z = half_sel(r[4].x, r[5].x, hi);
+      pk.w = half_sel(r[6].x, r[7].x, hi);
+    } else {
+      pk.x = half_sel(r[0].y, r[1].y, hi);
+      pk.y = half_sel(r[2].y, r[3].y, hi);
+      pk.z = half_sel(r[4].y, r[5].y, hi);
+      pk.w = half_sel(r[6].y, r[7].y, hi);
+    }
+    const bf16x8 bt = __builtin_bit_cast(bf16x8, pk);
+    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bt, acc[t], 0, 0, 0);
+  }
+}
+
+// Column ids (two int4) and A operand (bf16x8) of one chunk for this lane.
+struct ChunkRegs {
+  int4 ka, kb;
+  bf16x8 a;
+};
+
+__device__ __forceinline__ ChunkRegs load_chunk_regs(const SpmmPlanDev& p, int c, int g, int lane) {
+  ChunkRegs r;
+  const int4* cp = reinterpret_cast<const int4*>(p.tc_cols + (int64_t)c * 32 + g * 8);
+  r.ka = cp[0];
+  r.kb = cp[1];
+  r.a = reinterpret_cast<const bf16x8*>(p.tc_vals)[(int64_t)c * 64 + lane];
+  return r;
+}
+
+__global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p, const float* __restrict__ B,
+                                                                        int64_t ldb, int N, float* __restrict__ C,
+                                                                        int64_t ldc, int nC, int wpg, bool vec) {
+  extern __shared__ uint2 bsh[];  // [K][16]: 4 BF16 per entry
+  constexpr int kWaves = kLdsThreads / 64;
+  constexpr int kDepth = 1;  // chunks fetched ahead per step
+  const int chunk = blockIdx.x % nC;
+  const int grp = blockIdx.x / nC;
+  const int n0 = chunk * 64;
+  const int K = p.K;
+  // Stage B[0:K, n0:n0+64] as BF16: eight independent float4 loads per
+  // thread in flight before the packed stores.
+  for (int base = threadIdx.x; base < K * 16; base += 8 * kLdsThreads) {
+    float4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + u * kLdsThreads;
+      v[u] = i < K * 16 ? load_b4(B, ldb, i >> 4, n0 + 4 * (i & 15), N, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + u * kLdsThreads;
+      if (i < K * 16) bsh[i] = make_uint2(bf16_pair(v[u].x, v[u].y), bf16_pair(v[u].z, v[u].w));
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  const int col = n0 + 4 * j;
+  // This wave's contiguous run of windows, balanced by chunk count.
+  const int wg_beg = grp * wpg;
+  const int wg_end = min(wg_beg + wpg, p.W_tc);
+  if (wg_beg >= wg_end) return;
+  const int cb = p.tc_ptr[wg_beg], ce = p.tc_ptr[wg_end];
+  auto window_at = [&](int target) {  // first window w in [wg_beg, wg_end] with tc_ptr[w] >= target
+    int lo = wg_beg, hi = wg_end;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (p.tc_ptr[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
+  const int w_beg = window_at(cb + (int)(((int64_t)(ce - cb) * wib) / kWaves));
+  const int w_end = window_at(cb + (int)(((int64_t)(ce - cb) * (wib + 1)) / kWaves));
+  if (w_beg >= w_end) return;
+  const int c_beg = p.tc_ptr[w_beg], c_end = p.tc_ptr[w_end];
+
+  f32x4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  int w = w_beg;
+  int w_next = p.tc_ptr[w + 1];
+  SlotInfo si = load_slots(p, w * 16 + g * 4);
+  ChunkRegs cur[kDepth];
+#pragma unroll
+  for (int u = 0; u < kDepth; ++u)
+    if (c_beg + u < c_end) cur[u] = load_chunk_regs(p, c_beg + u, g, lane);
+  for (int c = c_beg; c < c_end; c += kDepth) {
+    ChunkRegs nxt[kDepth];
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u)
+      if (c + kDepth + u < c_end) nxt[u] = load_chunk_regs(p, c + kDepth + u, g, lane);
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) {
+      const int cc = c + u;
+      if (cc >= c_end) break;
+      while (cc >= w_next) {  // window w is complete: add its sparse part, store
+        const SlotInfo si_next = load_slots(p, (w + 1) * 16 + g * 4);
+        const int w_next2 = p.tc_ptr[w + 2];
+        finish_window(p, B, ldb, N, C, ldc, col, vec, si, acc);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        si = si_next;
+        ++w;
+        w_next = w_next2;
+      }
+      lds_chunk_mfma(bsh, j, cur[u].ka, cur[u].kb, cur[u].a, acc);
+    }
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) cur[u] = nxt[u];
+  }
+  finish_window(p, B, ldb, N, C, ldc, col, vec, si, acc);
+}
+
+// Remaining pieces of long sparse rows: lane group g of the wave takes
+// nonzeros q = beg + g, beg + g + 4, ...; the four partial float4 are reduced
+// across groups and added to C with atomicAdd (C already holds the window
+// pass's value of the row).
+__global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, const float* __restrict__ B,
+                                                             int64_t ldb, int N, float* __restrict__ C,
+                                                             int64_t ldc, int nC, bool vec) {
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int pc = wave / nC;
+  if (pc >= p.npieces) return;
+  const int n0 = (wave - pc * nC) * 64;
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const int col = n0 + 4 * (lane & 15);
+  const int beg = p.piece_beg[pc], end = p.piece_end[pc];
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f), cc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int q = beg + g; q < end; q += 4) {
+    const int k0 = p.res_cols[q];
+    const float v0 = p.res_vals[q];
+    const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
+    neu_add(s.x, cc.x, v0 * b0.x);
+    neu_add(s.y, cc.y, v0 * b0.y);
+    neu_add(s.z, cc.z, v0 * b0.z);
+    neu_add(s.w, cc.w, v0 * b0.w);
+  }
+  s.x += cc.x; s.y += cc.y; s.z += cc.z; s.w += cc.w;
+#pragma unroll
+  for (int off = 16; off <= 32; off <<= 1) {
+    s.x += __shfl_xor(s.x, off);
+    s.y += __shfl_xor(s.y, off);
+    s.z += __shfl_xor(s.z, off);
+    s.w += __shfl_xor(s.w, off);
+  }
+  if (g == 0) {
+    float* cp = C + (int64_t)p.piece_row[pc] * ldc + col;
+    if (col < N) atomicAdd(cp, s.x);
+    if (col + 1 < N) atomicAdd(cp + 1, s.y);
+    if (col + 2 < N) atomicAdd(cp + 2, s.z);
+    if (col + 3 < N) atomicAdd(cp + 3, s.w);
+  }
+}
+
+void prepare_spmm_kernels(const SpmmPlanDev& p) {
+  if (p.blk_dtype == 1 && p.K <= kLdsMaxK && p.K * 128 > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, p.K * 128);
+}
+
+static int spmm_kernel_choice() {
+  static int choice = -1;  // 0 auto, 1 streaming, 2 LDS
+  if (choice < 0) {
+    const char* e = getenv("STILE_SPMM_KERNEL");
+    choice = 0;
+    if (e && !strcmp(e, "stream")) choice = 1;
+    if (e && !strcmp(e, "lds")) choice = 2;
+  }
+  return choice;
+}
+
+hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, int N, float* C, int64_t ldc,
+                           hipStream_t s) {
   if (p.W == 0 || N == 0) return hipSuccess;
   const bool vec = (N % 4 == 0) && (ldb % 4 == 0) && (ldc % 4 == 0) &&
-                   ((reinterpret_cast<uintptr_t>(B) & 15) == 0) &&
-                   ((reinterpret_cast<uintptr_t>(C) & 15) == 0);
+                   ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && ((reinterpret_cast<uintptr_t>(C) & 15) == 0);
   const int nC = (N + 63) / 64;
-  const int64_t waves = (int64_t)p.W * nC;
-  dim3 grid((unsigned)((waves + 3) / 4));
-  switch (p.blk_dtype) {
-    case 1:
-      hipLaunchKernelGGL(hyb_spmm_kernel<1>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, vec);
-      break;
-    case 2:
-      hipLaunchKernelGGL(hyb_spmm_kernel<2>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, vec);
-      break;
-    default:
-      hipLaunchKernelGGL(hyb_spmm_kernel<0>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, vec);
-      break;
+  // B-stationary kernel when the staged rows (groups x K per column chunk)
+  // are fewer than the rows the streaming kernel would gather (32 per chunk).
+  const int groups = std::max(1, (256 + nC - 1) / nC);
+  const int choice = spmm_kernel_choice();
+  bool lds = p.blk_dtype == 1 && p.K <= kLdsMaxK && p.W_tc > 0;
+  if (choice == 1) lds = false;
+  if (choice == 2) lds = p.blk_dtype == 1 && p.K <= kLdsMaxK && p.W_tc > 0;
+  int64_t wave_base = 0;
+  if (lds) {
+    const int g = std::min(groups, p.W_tc);
+    const int wpg = (p.W_tc + g - 1) / g;
+    const int grid = nC * ((p.W_tc + wpg - 1) / wpg);
+    hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel, dim3(grid), dim3(kLdsThreads), (size_t)p.K * 128, s, p, B, ldb, N, C,
+                       ldc, nC, wpg, vec);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    wave_base = (int64_t)p.W_tc * nC;
   }
-  return hipGetLastError();
+  const int64_t waves = (int64_t)p.W_tc * nC + (int64_t)(p.W - p.W_tc) * nC * 4 - wave_base;
+  if (waves > 0) {
+    dim3 grid((unsigned)((waves + 3) / 4));
+    switch (p.blk_dtype) {
+      case 1:
+        hipLaunchKernelGGL(hyb_spmm_kernel<1>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, (int)wave_base, vec);
+        break;
+      case 2:
+        hipLaunchKernelGGL(hyb_spmm_kernel<2>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, (int)wave_base, vec);
+        break;
+      default:
+        hipLaunchKernelGGL(hyb_spmm_kernel<0>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, (int)wave_base, vec);
+        break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (p.npieces > 0) {
+    const int64_t fw = (int64_t)p.npieces * nC;
+    hipLaunchKernelGGL(hyb_spmm_fixup_kernel, dim3((unsigned)((fw + 3) / 4)), dim3(256), 0, s, p, B, ldb, N, C,
+                       ldc, nC, vec);
+    return hipGetLastError();
+  }
+  return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------

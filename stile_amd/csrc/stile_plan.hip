@@ -280,9 +280,44 @@ int add_spmm_pack(SpmmBuild& bd, const StileSpmmPack* pk, int32_t roff, int32_t 
 
 int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const int32_t W = (int32_t)(bd.win_rows.size() / 16);
-  const int64_t Mtot = (int64_t)bd.row_win.size();
-  // Global row -> (window, slot): row_win holds pack-local windows already
-  // offset (windows are appended in pack order), see add_spmm_pack.
+
+  // Dense columns of each window: merge duplicates (a column covered by
+  // several tiles of the window), sort by column.
+  for (int32_t w = 0; w < W; ++w) {
+    auto& cols = bd.win_cols[w];
+    std::sort(cols.begin(), cols.end(), [](const DenseCol& a, const DenseCol& b) { return a.col < b.col; });
+    size_t o = 0;
+    for (size_t i = 0; i < cols.size(); ++i) {
+      if (o > 0 && cols[o - 1].col == cols[i].col) {
+        for (int r = 0; r < 16; ++r) cols[o - 1].v[r] += cols[i].v[r];
+      } else {
+        cols[o++] = cols[i];
+      }
+    }
+    cols.resize(o);
+  }
+  // Windows with dense chunks first (W_tc of them), sparse-only windows after;
+  // the kernels map the two kinds onto waves differently.
+  std::vector<int32_t> order(W);
+  for (int32_t w = 0; w < W; ++w) order[w] = w;
+  std::stable_partition(order.begin(), order.end(), [&](int32_t w) { return !bd.win_cols[w].empty(); });
+  int32_t W_tc = 0;
+  while (W_tc < W && !bd.win_cols[order[W_tc]].empty()) ++W_tc;
+  {
+    std::vector<int32_t> new_of(W);
+    for (int32_t nw = 0; nw < W; ++nw) new_of[order[nw]] = nw;
+    std::vector<int32_t> rows2(bd.win_rows.size());
+    std::vector<std::vector<DenseCol>> cols2(W);
+    for (int32_t nw = 0; nw < W; ++nw) {
+      for (int r = 0; r < 16; ++r) rows2[(size_t)nw * 16 + r] = bd.win_rows[(size_t)order[nw] * 16 + r];
+      cols2[nw].swap(bd.win_cols[order[nw]]);
+    }
+    bd.win_rows.swap(rows2);
+    bd.win_cols.swap(cols2);
+    for (auto& rw : bd.row_win)
+      if (rw >= 0) rw = new_of[rw];
+  }
+
   // Sparse part as CSR over window slots, columns ascending within a slot.
   std::vector<int64_t> cnt((size_t)W * 16 + 1, 0);
   std::vector<int32_t> slot_of(bd.s_row.size());
@@ -321,28 +356,28 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
       }
     }
   }
+  // Long rows: the window pass takes the first kSplitNnz nonzeros of a slot,
+  // the rest is cut into pieces of kSplitNnz for the fix-up kernel.
   int32_t hub_rows = 0;
-  for (size_t sl = 0; sl + 1 < cnt.size(); ++sl)
-    if (cnt[sl + 1] - cnt[sl] > stile::kHubNnz) hub_rows++;
+  std::vector<int32_t> piece_row, piece_beg, piece_end;
+  for (size_t sl = 0; sl + 1 < cnt.size(); ++sl) {
+    const int64_t a = cnt[sl], b = cnt[sl + 1];
+    if (b - a > stile::kHubNnz) hub_rows++;
+    if (b - a <= stile::kSplitNnz) continue;
+    const int32_t row = bd.win_rows[sl];
+    for (int64_t q = a + stile::kSplitNnz; q < b; q += stile::kSplitNnz) {
+      piece_row.push_back(row);
+      piece_beg.push_back((int32_t)q);
+      piece_end.push_back((int32_t)std::min<int64_t>(b, q + stile::kSplitNnz));
+    }
+  }
 
-  // Dense chunks: merge duplicate columns of a window, sort by column, cut
-  // into chunks of 32 k-slots (the last one padded with value-0 slots that
-  // point at the chunk's first column).
+  // Dense chunks: cut each window's columns into chunks of 32 k-slots (the
+  // last one padded with value-0 slots that point at the chunk's first column).
   std::vector<int32_t> tc_ptr(W + 1, 0);
   int64_t nchunks = 0;
   for (int32_t w = 0; w < W; ++w) {
-    auto& cols = bd.win_cols[w];
-    std::sort(cols.begin(), cols.end(), [](const DenseCol& a, const DenseCol& b) { return a.col < b.col; });
-    size_t o = 0;
-    for (size_t i = 0; i < cols.size(); ++i) {
-      if (o > 0 && cols[o - 1].col == cols[i].col) {
-        for (int r = 0; r < 16; ++r) cols[o - 1].v[r] += cols[i].v[r];
-      } else {
-        cols[o++] = cols[i];
-      }
-    }
-    cols.resize(o);
-    nchunks += (int64_t)(cols.size() + 31) / 32;
+    nchunks += (int64_t)(bd.win_cols[w].size() + 31) / 32;
     tc_ptr[w + 1] = (int32_t)nchunks;
   }
   if (nchunks > INT32_MAX / 64) return fail(STILE_E_UNSUPPORTED, "too many dense chunks");
@@ -399,6 +434,9 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const size_t o_resptr = lay.add(res_ptr.size() * 4);
   const size_t o_rescols = lay.add(res_cols.size() * 4);
   const size_t o_resvals = lay.add(res_vals.size() * 4);
+  const size_t o_prow = lay.add(piece_row.size() * 4);
+  const size_t o_pbeg = lay.add(piece_beg.size() * 4);
+  const size_t o_pend = lay.add(piece_end.size() * 4);
   HostBlob blob;
   blob.data.assign(lay.bytes, 0);
   blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
@@ -409,6 +447,9 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   blob.put(o_resptr, res_ptr.data(), res_ptr.size() * 4);
   blob.put(o_rescols, res_cols.data(), res_cols.size() * 4);
   blob.put(o_resvals, res_vals.data(), res_vals.size() * 4);
+  blob.put(o_prow, piece_row.data(), piece_row.size() * 4);
+  blob.put(o_pbeg, piece_beg.data(), piece_beg.size() * 4);
+  blob.put(o_pend, piece_end.data(), piece_end.size() * 4);
 
   StilePlan* plan = new (std::nothrow) StilePlan();
   if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
@@ -424,7 +465,12 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.M = bd.M;
   d.K = bd.K;
   d.W = W;
+  d.W_tc = W_tc;
   d.nchunks = (int32_t)nchunks;
+  d.npieces = (int32_t)piece_row.size();
+  d.piece_row = dptr<int32_t>(plan, o_prow);
+  d.piece_beg = dptr<int32_t>(plan, o_pbeg);
+  d.piece_end = dptr<int32_t>(plan, o_pend);
   d.blk_dtype = blk_dtype;
   d.win_rows = dptr<int32_t>(plan, o_rows);
   d.tc_ptr = dptr<int32_t>(plan, o_tcptr);
@@ -445,8 +491,8 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   in.sparse_nnz = nres;
   in.device_bytes = (int64_t)plan->dev_bytes;
   in.hub_rows = hub_rows;
-  in.num_split = 0;
-  (void)Mtot;
+  in.num_split = (int32_t)piece_row.size();
+  stile::prepare_spmm_kernels(d);
   *out = plan;
   return STILE_OK;
 }
