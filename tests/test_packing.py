@@ -1,0 +1,134 @@
+"""CPU tests: the product's packed buffers (prepare_inputs_BSPMM/BSDDMM
+layouts) against the oracle's transcription of the reference packers,
+bit-exact, on tile sets from the product search; and the packs' invariants
+(re-expanding them reproduces A exactly, each nonzero once)."""
+import json
+
+import numpy as np
+import pytest
+import scipy.sparse
+
+from oracle import formats as of
+from stile_amd import api, datasets, my_fuse_formats
+from stile_amd.gen_formats_v2 import get_template_str
+
+
+def _cases():
+    out = []
+    A = datasets.pruned_layer(seed=5, M=256, K=128, N=8)[0]
+    out.append(("pruned_values", A, api.SearchOptions()))
+    out.append(("pruned_ones", scipy.sparse.csr_matrix((np.ones(A.nnz, np.float32), A.indices, A.indptr),
+                                                      shape=A.shape), api.SearchOptions()))
+    out.append(("pruned_only_ell", A, api.SearchOptions(only_ELL=True)))
+    out.append(("pruned_single_level", A, api.SearchOptions(max_level_num=1)))
+    out.append(("rmat9", datasets.rmat_graph(9, seed=2), api.SearchOptions()))
+    out.append(("logsparse", datasets.log_sparse_mask(300), api.SearchOptions()))
+    B = scipy.sparse.random(200, 150, density=0.05, format="csr", random_state=3, dtype=np.float32).tolil()
+    B[7, :] = 1.0
+    B[9, :] = 0.0
+    out.append(("random_hub", B.tocsr(), api.SearchOptions(max_bucket_size=8)))
+    out.append(("random_nowithdraw", B.tocsr(), api.SearchOptions(no_withdraw=True, max_bucket_size=8)))
+    return out
+
+
+def expand_spmm_pack(pack):
+    """COO of every nonzero value in the packed groups (ELL rows at the
+    sentinel M excluded)."""
+    M, K = pack["M"], pack["K"]
+    rows, cols, vals = [], [], []
+    for g in pack["ell"]:
+        I, Kc, T = g["m_blk"], g["nnz_cols"], g["num_tiles"]
+        a = g["a"].reshape(T * I, Kc)
+        r = np.repeat(g["indices_reordered_i"], Kc).reshape(T * I, Kc)
+        m = (a != 0) & (r != M)
+        rows.append(r[m]); cols.append(g["indices_k"][m]); vals.append(a[m])
+    for g in pack["tc"]:
+        T, Kg = g["nnzb"], g["group_size"]
+        win_of_tile = np.repeat(np.arange(g["num_windows"]), np.diff(g["indptr"]))
+        for t in range(T):
+            s = win_of_tile[t]
+            rr = g["i_indices"][s] if g["rows_2d"] else g["i_indices"][s] * 16 + np.arange(16)
+            cc = g["indices"][t] * 16 + np.arange(16) if g["k_notsorted"] else g["indices"][t]
+            m = g["a"][t] != 0
+            ri, ci = np.nonzero(m)
+            rows.append(np.asarray(rr)[ri]); cols.append(np.asarray(cc)[ci]); vals.append(g["a"][t][m])
+    rows, cols, vals = (np.concatenate(x) if x else np.zeros(0) for x in (rows, cols, vals))
+    return rows, cols, vals
+
+
+@pytest.mark.parametrize("name,A,opts", _cases())
+def test_spmm_pack_matches_oracle_and_reexpands(name, A, opts):
+    pack, tiles, op = api.spmm_pack(A, opts)
+    formats = my_fuse_formats.get_formats_from_selected_tiles(tiles, ['A'], opts.dsize)
+    gi = {"ell": 0, "tc": 0}
+    for fmt, ftiles in formats.items():
+        params = json.loads(fmt[2])
+        I, K = fmt[1][0][1], fmt[1][2][1]
+        if fmt[0] == "sparse_template_ell":
+            mine = pack["ell"][gi["ell"]]
+            gi["ell"] += 1
+            ref = of.pack_ell_group(ftiles, I, K)
+            assert np.array_equal(mine["a"], ref["a"])
+            assert np.array_equal(mine["indices_k"], ref["indices_k"])
+            assert np.array_equal(mine["indices_reordered_i"], ref["indices_reordered_i"])
+        else:
+            mine = pack["tc"][gi["tc"]]
+            gi["tc"] += 1
+            rows_2d = bool(params["idx_reordered"][0] or params["real_atomic"])
+            ref = of.pack_tc_group(ftiles, I, K, rows_2d)
+            assert mine["rows_2d"] == rows_2d
+            for k in ("a", "i_indices", "indptr", "indices"):
+                assert np.array_equal(mine[k], ref[k]), k
+    # every nonzero exactly once, with its value
+    r, c, v = expand_spmm_pack(pack)
+    got = scipy.sparse.coo_matrix((v, (r, c)), shape=A.shape)
+    assert len(r) == A.nnz
+    key = np.asarray(r, np.int64) * A.shape[1] + np.asarray(c, np.int64)
+    assert len(np.unique(key)) == A.nnz
+    diff = (got.tocsr() - scipy.sparse.csr_matrix(A, dtype=np.float32))
+    assert abs(diff).max() == 0 if diff.nnz else True
+
+
+@pytest.mark.parametrize("scale,Kf", [(9, 32), (10, 64)])
+def test_sddmm_pack_matches_oracle(scale, Kf):
+    A = datasets.rmat_graph(scale, seed=scale)
+    pack, tiles, op = api.sddmm_pack(A, Kf=Kf)
+    formats = my_fuse_formats.get_formats_from_selected_tiles(tiles, ['A'], 16)
+    seen = np.zeros(A.nnz, bool)
+    i1 = it = 0
+    for fmt, ftiles in formats.items():
+        if fmt[0] == "1D_sddmm":
+            op1 = ftiles[0].op
+            rows, cols, ids = of.sddmm_1d_ids(A, op1.idx_values_list[0], op1.inps[0])
+            assert np.array_equal(op1.ori_row_ids_1d, rows)
+            assert np.array_equal(op1.ori_col_ids_1d, cols)
+            assert np.array_equal(op1.nnz_id_matrix, ids)
+            ref = of.pack_sddmm_1d_group(ftiles, fmt[1][0])
+            mine = pack["g1d"][i1]
+            i1 += 1
+            for k in ("mid_idx", "j_idx", "remap"):
+                assert np.array_equal(mine[k], ref[k]), k
+            seen[mine["remap"]] = True
+            assert np.array_equal(A.indices[mine["remap"]], mine["j_idx"])
+        else:
+            ref = of.pack_sddmm_tc_group(A, ftiles, fmt[1])
+            mine = pack["tc"][it]
+            it += 1
+            for k in ("i_indices", "j_indices", "indptr", "indices_dense", "indices"):
+                assert np.array_equal(mine[k], ref[k]), k
+            seen[mine["indices"]] = True
+    assert seen.all(), "some output entries are computed by no tile"
+
+
+def test_set_params_scalars():
+    A = datasets.pruned_layer(seed=5, M=256, K=128, N=8)[0]
+    pack, tiles, op = api.spmm_pack(A, N=100)
+    formats = my_fuse_formats.get_formats_from_selected_tiles(tiles, ['A'], 16)
+    p = my_fuse_formats.set_params_BSPMM(formats, 16, op)
+    # m_tot = max(ceil(M/16)*16, M+1) and n_tot = padded N (reference my_fuse_formats.py:2797-2822)
+    assert p["m_tot"] == 257
+    assert p["n_tot"] == of.get_padded_j_TC(100, 16) == 128
+    assert p["k_tot"] == 128
+    for g in p["groups"]:
+        if g["template"] == "TensorCore_template":
+            assert g["tile_size"] == 16 and g["funit"] == 32 and g["nb"] == -(-128 // g["group_size"])
