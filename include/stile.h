@@ -64,6 +64,9 @@ typedef struct {
   const float* a;
   const int32_t* indices_k;
   const int32_t* indices_reordered_i;
+  const int32_t* a_ids; /* optional [T][I*K]: index of each slot's value in A's
+                           CSR value array, -1 for padding; lets
+                           stile_plan_set_values refresh the values (NULL: static) */
 } StileEllGroup;
 
 /* One "TensorCore_template" group (my_fuse_formats.py:3424-3504).
@@ -86,6 +89,7 @@ typedef struct {
   const int32_t* i_indices;
   const int32_t* indptr;
   const int32_t* indices;
+  const int32_t* a_ids; /* optional [nnzb][16][K], as StileEllGroup.a_ids */
 } StileTcGroup;
 
 typedef struct {
@@ -211,13 +215,21 @@ int stile_plan_create_spmm_batched(const StileSpmmPack* packs, int32_t count,
 int stile_plan_create_sddmm_batched(const StileSddmmPack* packs, int32_t count,
                                     int32_t blk_dtype, StilePlan** out);
 
+/* Replace the plan's matrix values by vals[0..nnz) (device pointer, A's CSR
+ * value order) without rebuilding the plan: every packed slot is refreshed
+ * from the value id it was built with.  Requires a plan whose groups all
+ * carried a_ids; FP32 and BF16 plans.  Not in the reference (its values are
+ * the 0/1 position space); used for training, where A's values change every
+ * step (the SDDMM backward multiplies by the incoming gradient). */
+int stile_plan_set_values(StilePlan* plan, const float* vals, int64_t nnz, void* stream);
+
 int stile_plan_info(const StilePlan* plan, StilePlanInfo* info);
 int stile_plan_destroy(StilePlan* plan);
 
 /* Message of the last failure on this thread ("" if none). */
 const char* stile_last_error(void);
 /* ABI version, bumped on any layout change of the structs above. */
-int32_t stile_abi_version(void);
+int32_t stile_abi_version(void);  /* 2 */
 
 #ifdef __cplusplus
 }

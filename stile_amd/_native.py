@@ -23,14 +23,14 @@ _f32p = ctypes.POINTER(ctypes.c_float)
 class StileEllGroup(ctypes.Structure):
     _fields_ = [("m_blk", ctypes.c_int32), ("nnz_cols", ctypes.c_int32),
                 ("num_tiles", ctypes.c_int32), ("real_atomic", ctypes.c_int32),
-                ("a", _f32p), ("indices_k", _i32p), ("indices_reordered_i", _i32p)]
+                ("a", _f32p), ("indices_k", _i32p), ("indices_reordered_i", _i32p), ("a_ids", _i32p)]
 
 
 class StileTcGroup(ctypes.Structure):
     _fields_ = [("group_size", ctypes.c_int32), ("num_windows", ctypes.c_int32),
                 ("nnzb", ctypes.c_int32), ("rows_2d", ctypes.c_int32),
                 ("k_notsorted", ctypes.c_int32), ("real_atomic", ctypes.c_int32),
-                ("a", _f32p), ("i_indices", _i32p), ("indptr", _i32p), ("indices", _i32p)]
+                ("a", _f32p), ("i_indices", _i32p), ("indptr", _i32p), ("indices", _i32p), ("a_ids", _i32p)]
 
 
 class StileSpmmPack(ctypes.Structure):
@@ -88,6 +88,7 @@ _SIGNATURES = {
     "stile_sddmm_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "stile_plan_set_values": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     "stile_plan_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(StilePlanInfo)]),
     "stile_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "stile_last_error": (ctypes.c_char_p, []),
@@ -219,19 +220,24 @@ class _PackArrays:
         self.keep.append(a)
         return _ptr(a, ctypes.c_float)
 
+    def i32_opt(self, a):
+        return None if a is None else self.i32(a)
+
 
 def _spmm_pack_struct(pack, arrays):
     """pack: dict with M, K, ell (list of dicts), tc (list of dicts); Appendix-A layouts."""
     ells = [StileEllGroup(m_blk=g["m_blk"], nnz_cols=g["nnz_cols"], num_tiles=g["num_tiles"],
                           real_atomic=int(bool(g.get("real_atomic", False))),
                           a=arrays.f32(g["a"]), indices_k=arrays.i32(g["indices_k"]),
-                          indices_reordered_i=arrays.i32(g["indices_reordered_i"]))
+                          indices_reordered_i=arrays.i32(g["indices_reordered_i"]),
+                          a_ids=arrays.i32_opt(g.get("a_ids")))
             for g in pack["ell"]]
     tcs = [StileTcGroup(group_size=g["group_size"], num_windows=g["num_windows"], nnzb=g["nnzb"],
                         rows_2d=int(bool(g["rows_2d"])), k_notsorted=int(bool(g.get("k_notsorted", False))),
                         real_atomic=int(bool(g.get("real_atomic", False))),
                         a=arrays.f32(g["a"]), i_indices=arrays.i32(g["i_indices"]),
-                        indptr=arrays.i32(g["indptr"]), indices=arrays.i32(g["indices"]))
+                        indptr=arrays.i32(g["indptr"]), indices=arrays.i32(g["indices"]),
+                        a_ids=arrays.i32_opt(g.get("a_ids")))
            for g in pack["tc"]]
     ell_arr = (StileEllGroup * max(len(ells), 1))(*ells)
     tc_arr = (StileTcGroup * max(len(tcs), 1))(*tcs)
@@ -289,6 +295,17 @@ class Plan:
                                                    ctypes.byref(h)))
         return cls(h, "sddmm", sum(p["M"] for p in packs), sum(p["N"] for p in packs),
                    sum(p["nnz"] for p in packs))
+
+    def set_values(self, vals):
+        """Refresh A's values (device float32, A's CSR order) without rebuilding."""
+        import torch
+        if self.kind != "spmm":
+            raise StileError("not an SpMM plan")
+        _dev_check(vals)
+        if vals.dtype != torch.float32:
+            raise StileError("values must be float32")
+        vals = vals.contiguous()
+        _check(load().stile_plan_set_values(self._h, vals.data_ptr(), vals.numel(), _stream(vals)))
 
     def info(self):
         inf = StilePlanInfo()

@@ -84,6 +84,12 @@ hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,
                             int64_t ldx, const float* Y, int64_t ldy, int Kf,
                             float* out, hipStream_t s);
 
+// Refresh a plan's packed values from A's value array: tc slot i takes
+// vals[tc_ids[i]] (0 when -1), converted to the plan's block type; sparse
+// entry j takes vals[res_ids[j]].
+hipError_t launch_set_values(const SpmmPlanDev& p, const float* vals, const int32_t* tc_ids, int64_t n_tc,
+                             const int32_t* res_ids, int64_t n_res, hipStream_t s);
+
 // Zero-fill kernels (used instead of hipMemsetAsync so that every *_run call
 // records only kernel nodes under HIP graph capture).
 hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t s);

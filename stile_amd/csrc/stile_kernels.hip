@@ -897,6 +897,41 @@ hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,
 }
 
 // ---------------------------------------------------------------------------
+// Value refresh.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void set_values_kernel(const float* __restrict__ vals,
+                                                         const int32_t* __restrict__ tc_ids, int64_t n_tc,
+                                                         void* tc_vals, int mode,
+                                                         const int32_t* __restrict__ res_ids, int64_t n_res,
+                                                         float* res_vals) {
+  const int64_t n = n_tc + n_res;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < n_tc) {
+      const int id = tc_ids[i];
+      const float v = id >= 0 ? vals[id] : 0.f;
+      if (mode == 1)
+        reinterpret_cast<__bf16*>(tc_vals)[i] = (__bf16)v;
+      else
+        reinterpret_cast<float*>(tc_vals)[i] = v;
+    } else {
+      const int64_t j = i - n_tc;
+      res_vals[j] = vals[res_ids[j]];
+    }
+  }
+}
+
+hipError_t launch_set_values(const SpmmPlanDev& p, const float* vals, const int32_t* tc_ids, int64_t n_tc,
+                             const int32_t* res_ids, int64_t n_res, hipStream_t s) {
+  const int64_t n = n_tc + n_res;
+  if (n == 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(set_values_kernel, dim3((unsigned)blocks), dim3(256), 0, s, vals, tc_ids, n_tc,
+                     const_cast<void*>(p.tc_vals), p.blk_dtype, res_ids, n_res, const_cast<float*>(p.res_vals));
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Zero fill.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void fill_zero_2d_kernel(float* __restrict__ p, int rows, int cols,

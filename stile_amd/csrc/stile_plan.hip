@@ -32,6 +32,12 @@ struct StilePlan {
   stile::SpmmPlanDev spmm{};
   stile::SddmmPlanDev sddmm{};
   StilePlanInfo info{};
+  // value refresh (stile_plan_set_values)
+  bool has_ids = false;
+  int64_t max_id = -1;
+  int64_t n_tc_slots = 0, n_res = 0;
+  const int32_t* tc_ids = nullptr;
+  const int32_t* res_ids = nullptr;
 };
 
 namespace {
@@ -106,7 +112,8 @@ uint16_t to_f16_bits(float f) {
 
 struct DenseCol {
   int32_t col;
-  float v[16];  // values for window slots 0..15
+  float v[16];    // values for window slots 0..15
+  int32_t id[16]; // value ids (index into A's value array), -1 = none
 };
 
 struct SpmmBuild {
@@ -118,18 +125,25 @@ struct SpmmBuild {
   // sparse part, as (row, col, val) triples
   std::vector<int32_t> s_row, s_col;
   std::vector<float> s_val;
+  std::vector<int32_t> s_id;
   int64_t block_nnz = 0;
+  bool has_ids = true;     // every group carried a_ids
+  int64_t max_id = -1;     // largest value id (values array must be longer)
 };
 
 // Validate one pack and append its windows / products to the build with row
 // offset roff and column offset coff (block-diagonal batching).
-int add_spmm_pack(SpmmBuild& bd, const StileSpmmPack* pk, int32_t roff, int32_t coff) {
+int add_spmm_pack(SpmmBuild& bd, const StileSpmmPack* pk, int32_t roff, int32_t coff, int64_t ioff) {
   if (!pk) return fail(STILE_E_INVALID, "null pack");
   if (pk->M < 0 || pk->K < 0) return fail(STILE_E_INVALID, "negative M or K");
   if ((pk->num_ell > 0 && !pk->ell) || (pk->num_tc > 0 && !pk->tc) || pk->num_ell < 0 || pk->num_tc < 0)
     return fail(STILE_E_INVALID, "group array missing");
   const int32_t M = pk->M, K = pk->K;
   const size_t base_w = bd.win_rows.size() / 16;
+  for (int gi = 0; gi < pk->num_tc; ++gi)
+    if (!pk->tc[gi].a_ids) bd.has_ids = false;
+  for (int gi = 0; gi < pk->num_ell; ++gi)
+    if (!pk->ell[gi].a_ids) bd.has_ids = false;
 
   // 1. windows from the TC groups, keyed by their first real row.
   std::unordered_map<int32_t, int32_t> key2win;  // first row -> window index
@@ -230,14 +244,23 @@ int add_spmm_pack(SpmmBuild& bd, const StileSpmmPack* pk, int32_t roff, int32_t 
           dc.col = col + coff;
           bool any = false;
           for (int r = 0; r < 16; ++r) {
-            float v = g.a[((int64_t)t * 16 + r) * Kg + kk];
-            dc.v[r] = v;
-            if (v != 0.f) {
+            const int64_t at = ((int64_t)t * 16 + r) * Kg + kk;
+            const float v = g.a[at];
+            const int32_t id = g.a_ids ? g.a_ids[at] : -1;
+            // a slot holds a nonzero when it has an id (id-carrying packs) or a
+            // nonzero value (value-only packs)
+            const bool present = g.a_ids ? id >= 0 : v != 0.f;
+            if (g.a_ids && id < -1) return fail(STILE_E_INVALID, "TC group: negative value id");
+            if (g.a_ids && !present && v != 0.f) return fail(STILE_E_INVALID, "TC group: value without an id");
+            dc.v[r] = present ? v : 0.f;
+            dc.id[r] = present && g.a_ids ? id + (int32_t)ioff : -1;
+            if (present) {
               if (win < 0) return fail(STILE_E_INVALID, "TC group: nonzero value in a window without rows");
               if (bd.win_rows[(size_t)win * 16 + r] < 0)
                 return fail(STILE_E_INVALID, "TC group: nonzero value in a padding row");
               any = true;
               bd.block_nnz++;
+              bd.max_id = std::max<int64_t>(bd.max_id, dc.id[r]);
             }
           }
           if (!any) continue;
@@ -262,14 +285,20 @@ int add_spmm_pack(SpmmBuild& bd, const StileSpmmPack* pk, int32_t roff, int32_t 
         const int32_t row = g.indices_reordered_i[tr];
         if (row < 0 || row > M) return fail(STILE_E_INVALID, "ELL group: row index out of range");
         for (int kk = 0; kk < Kc; ++kk) {
-          const float v = g.a[t * (int64_t)I * Kc + (int64_t)i * Kc + kk];
-          if (v == 0.f) continue;   // padding slots and explicit zeros add nothing
+          const int64_t at = t * (int64_t)I * Kc + (int64_t)i * Kc + kk;
+          const float v = g.a[at];
+          const int32_t id = g.a_ids ? g.a_ids[at] : -1;
+          if (g.a_ids && id < -1) return fail(STILE_E_INVALID, "ELL group: negative value id");
+          const bool present = g.a_ids ? id >= 0 : v != 0.f;
+          if (!present) continue;   // padding slots (and explicit zeros of value-only packs)
           if (row == M) continue;   // sentinel row: products are discarded
           const int32_t col = g.indices_k[tr * Kc + kk];
           if (col < 0 || col >= K) return fail(STILE_E_INVALID, "ELL group: column index out of range");
           bd.s_row.push_back(row + roff);
           bd.s_col.push_back(col + coff);
           bd.s_val.push_back(v);
+          bd.s_id.push_back(g.a_ids ? id + (int32_t)ioff : -1);
+          if (g.a_ids) bd.max_id = std::max<int64_t>(bd.max_id, id + ioff);
         }
       }
     }
@@ -289,7 +318,12 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
     size_t o = 0;
     for (size_t i = 0; i < cols.size(); ++i) {
       if (o > 0 && cols[o - 1].col == cols[i].col) {
-        for (int r = 0; r < 16; ++r) cols[o - 1].v[r] += cols[i].v[r];
+        for (int r = 0; r < 16; ++r) {
+          if (cols[o - 1].id[r] >= 0 && cols[i].id[r] >= 0)
+            return fail(STILE_E_INVALID, "two TC tiles cover the same nonzero");
+          cols[o - 1].v[r] += cols[i].v[r];
+          if (cols[i].id[r] >= 0) cols[o - 1].id[r] = cols[i].id[r];
+        }
       } else {
         cols[o++] = cols[i];
       }
@@ -333,27 +367,22 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   std::vector<int32_t> res_ptr(cnt.begin(), cnt.end());
   std::vector<int32_t> res_cols(nres);
   std::vector<float> res_vals(nres);
+  std::vector<int32_t> res_ids(bd.has_ids ? nres : 0);
   {
     std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-    for (size_t e = 0; e < bd.s_row.size(); ++e) {
-      const int64_t at = fill[slot_of[e]]++;
-      res_cols[at] = bd.s_col[e];
-      res_vals[at] = bd.s_val[e];
-    }
-    std::vector<std::pair<int32_t, float>> tmp;
+    std::vector<int64_t> src(nres);
+    for (size_t e = 0; e < bd.s_row.size(); ++e) src[fill[slot_of[e]]++] = (int64_t)e;
+    // columns ascending within a slot (stable: ties keep pack order)
     for (size_t sl = 0; sl + 1 < cnt.size(); ++sl) {
       const int64_t a = cnt[sl], b = cnt[sl + 1];
-      if (b - a < 2) continue;
-      tmp.resize(b - a);
-      for (int64_t q = a; q < b; ++q) tmp[q - a] = {res_cols[q], res_vals[q]};
-      std::stable_sort(tmp.begin(), tmp.end(),
-                       [](const std::pair<int32_t, float>& x, const std::pair<int32_t, float>& y) {
-                         return x.first < y.first;
-                       });
-      for (int64_t q = a; q < b; ++q) {
-        res_cols[q] = tmp[q - a].first;
-        res_vals[q] = tmp[q - a].second;
-      }
+      if (b - a >= 2)
+        std::stable_sort(src.begin() + a, src.begin() + b,
+                         [&](int64_t x, int64_t y) { return bd.s_col[x] < bd.s_col[y]; });
+    }
+    for (int64_t q = 0; q < nres; ++q) {
+      res_cols[q] = bd.s_col[src[q]];
+      res_vals[q] = bd.s_val[src[q]];
+      if (bd.has_ids) res_ids[q] = bd.s_id[src[q]];
     }
   }
   // Long rows: the window pass takes the first kSplitNnz nonzeros of a slot,
@@ -385,6 +414,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   std::vector<int32_t> tc_cols((size_t)nchunks * 32);
   std::vector<unsigned char> tc_vals((size_t)nchunks * 64 * 8 * val_bytes);
   std::vector<float> tc_scale(blk_dtype == STILE_BLK_F16 ? nchunks : 0);
+  std::vector<int32_t> tc_ids(bd.has_ids ? (size_t)nchunks * 64 * 8 : 0);
   for (int32_t w = 0; w < W; ++w) {
     const auto& cols = bd.win_cols[w];
     for (int64_t c = tc_ptr[w]; c < tc_ptr[w + 1]; ++c) {
@@ -411,6 +441,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
           const size_t ks = 4 * e + g;
           const float v = base + ks < cols.size() ? cols[base + ks].v[row] : 0.f;
           const size_t at = ((size_t)c * 64 + lane) * 8 + e;
+          if (bd.has_ids) tc_ids[at] = base + ks < cols.size() ? cols[base + ks].id[row] : -1;
           if (blk_dtype == STILE_BLK_F32) {
             std::memcpy(&tc_vals[at * 4], &v, 4);
           } else if (blk_dtype == STILE_BLK_BF16) {
@@ -437,6 +468,8 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const size_t o_prow = lay.add(piece_row.size() * 4);
   const size_t o_pbeg = lay.add(piece_beg.size() * 4);
   const size_t o_pend = lay.add(piece_end.size() * 4);
+  const size_t o_tcids = lay.add(tc_ids.size() * 4);
+  const size_t o_resids = lay.add(res_ids.size() * 4);
   HostBlob blob;
   blob.data.assign(lay.bytes, 0);
   blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
@@ -450,6 +483,8 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   blob.put(o_prow, piece_row.data(), piece_row.size() * 4);
   blob.put(o_pbeg, piece_beg.data(), piece_beg.size() * 4);
   blob.put(o_pend, piece_end.data(), piece_end.size() * 4);
+  blob.put(o_tcids, tc_ids.data(), tc_ids.size() * 4);
+  blob.put(o_resids, res_ids.data(), res_ids.size() * 4);
 
   StilePlan* plan = new (std::nothrow) StilePlan();
   if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
@@ -480,6 +515,12 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.res_ptr = dptr<int32_t>(plan, o_resptr);
   d.res_cols = dptr<int32_t>(plan, o_rescols);
   d.res_vals = dptr<float>(plan, o_resvals);
+  plan->has_ids = bd.has_ids;
+  plan->max_id = bd.max_id;
+  plan->n_tc_slots = (int64_t)tc_ids.size();
+  plan->n_res = nres;
+  plan->tc_ids = bd.has_ids ? dptr<int32_t>(plan, o_tcids) : nullptr;
+  plan->res_ids = bd.has_ids ? dptr<int32_t>(plan, o_resids) : nullptr;
   StilePlanInfo& in = plan->info;
   in.kind = 1;
   in.blk_dtype = blk_dtype;
@@ -504,11 +545,15 @@ int create_spmm(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype, St
   if (blk_dtype < STILE_BLK_F32 || blk_dtype > STILE_BLK_F16)
     return fail(STILE_E_INVALID, "unknown blk_dtype");
   SpmmBuild bd;
-  int64_t roff = 0, coff = 0;
+  int64_t roff = 0, coff = 0, ioff = 0;
   for (int32_t i = 0; i < count; ++i) {
     const size_t w0 = bd.win_rows.size() / 16;
     const size_t r0 = bd.row_win.size();
-    int rc = add_spmm_pack(bd, &packs[i], (int32_t)roff, (int32_t)coff);
+    // value ids of matrix i are offset past the largest id of the previous
+    // ones: the batched values array is the concatenation of the matrices'
+    // value arrays, each as long as its largest id + 1
+    ioff = bd.max_id + 1;
+    int rc = add_spmm_pack(bd, &packs[i], (int32_t)roff, (int32_t)coff, ioff);
     if (rc != STILE_OK) return rc;
     (void)w0;
     (void)r0;
@@ -668,7 +713,7 @@ int create_sddmm(const StileSddmmPack* packs, int32_t count, int32_t blk_dtype, 
 
 extern "C" {
 
-int32_t stile_abi_version(void) { return 1; }
+int32_t stile_abi_version(void) { return 2; }
 
 const char* stile_last_error(void) { return g_err.c_str(); }
 
@@ -745,6 +790,17 @@ int stile_sddmm_run(const StilePlan* plan, const float* X, int64_t ldx, const fl
   }
   hipError_t e = stile::launch_hyb_sddmm(plan->sddmm, X, ldx, Y, ldy, Kf, out, (hipStream_t)stream);
   return e == hipSuccess ? STILE_OK : hip_fail(e, "hyb_sddmm launch");
+}
+
+int stile_plan_set_values(StilePlan* plan, const float* vals, int64_t nnz, void* stream) {
+  if (!plan || plan->kind != 1) return fail(STILE_E_INVALID, "not an SpMM plan");
+  if (!plan->has_ids) return fail(STILE_E_INVALID, "plan was built without value ids (a_ids)");
+  if (plan->blk_dtype == STILE_BLK_F16) return fail(STILE_E_UNSUPPORTED, "value refresh of FP16 plans");
+  if (nnz <= plan->max_id) return fail(STILE_E_INVALID, "values array shorter than the plan's largest value id");
+  if (!vals && (plan->n_tc_slots + plan->n_res) > 0) return fail(STILE_E_INVALID, "null values");
+  hipError_t e = stile::launch_set_values(plan->spmm, vals, plan->tc_ids, plan->n_tc_slots, plan->res_ids,
+                                          plan->n_res, (hipStream_t)stream);
+  return e == hipSuccess ? STILE_OK : hip_fail(e, "set_values launch");
 }
 
 int stile_plan_info(const StilePlan* plan, StilePlanInfo* info) {

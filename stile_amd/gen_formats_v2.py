@@ -309,7 +309,8 @@ class KOrder(object):
     def __init__(self, head, K):
         self.head = np.asarray(head, dtype=np.int64)
         self.K = int(K)
-        self._sorted = np.sort(self.head)
+        self._order = np.argsort(self.head, kind='stable')
+        self._sorted = self.head[self._order]
         self._gap = self._sorted - np.arange(len(self._sorted))   # empties before each head column
 
     def __len__(self):
@@ -340,11 +341,10 @@ class KOrder(object):
         cols = np.asarray(cols, dtype=np.int64)
         n = len(self.head)
         if n:
-            order = np.argsort(self.head, kind='stable')
-            loc = np.searchsorted(self.head[order], cols)
+            loc = np.searchsorted(self._sorted, cols)
             loc_c = np.minimum(loc, n - 1)
-            hit = self.head[order][loc_c] == cols
-            head_pos = order[loc_c]
+            hit = self._sorted[loc_c] == cols
+            head_pos = self._order[loc_c]
         else:
             hit = np.zeros(len(cols), bool)
             head_pos = np.zeros(len(cols), np.int64)
@@ -837,7 +837,8 @@ class ComputeTile(object):
         e = np.arange(lo, hi)
         if tpl in ('TensorCore_template', 'TC_sddmm'):
             w = self.tile_pos[0]
-            cp = _tc_condensed_pos(op, w, op._ent_col[e])
+            cpos = getattr(op, "_ent_cpos", None)
+            cp = cpos[e] if cpos is not None else _tc_condensed_pos(op, w, op._ent_col[e])
             e = e[(cp >= self.tile_k_rng[0]) & (cp <= self.tile_k_rng[1])]
         return e
 
@@ -847,7 +848,7 @@ class ComputeTile(object):
 
     def update_nnz_uncovered(self):
         e = self.entries()
-        self.nnz_uncovered = self.j_num * int(np.count_nonzero(self.op.uncovered_mask()[e]))
+        self.nnz_uncovered = self.j_num * int(np.count_nonzero(self.op.state.uncovered[self.op._ent_id[e]]))
 
     def set_avg_cost(self):
         if self.cost is None:

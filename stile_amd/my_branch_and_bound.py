@@ -128,7 +128,7 @@ def _cover(tile, entries=None, full=False):
     op = tile.op
     e = tile.entries() if entries is None else entries
     if not full:
-        e = e[op.uncovered_mask()[e]]
+        e = e[op.state.uncovered[op._ent_id[e]]]
     ids = op._ent_id[e]
     tile.covered_ids = ids
     tile.covered_rows = op._ent_row[e]

@@ -172,9 +172,10 @@ def pack_ell_group(tiles, tile_sizes):
     K = math.prod(tile_sizes[2][1:])
     T = len(tiles)
     a = np.zeros((T * I, K), np.float32)
+    a_ids = np.full((T * I, K), -1, np.int32)
     indices_k = np.empty((T * I, K), np.int32)
     ir = np.empty(T * I, np.int32)
-    t_of, r_of, c_of, v_of = [], [], [], []
+    t_of, r_of, c_of, v_of, i_of = [], [], [], [], []
     for c, t in enumerate(tiles):
         iv = t.op.idx_values_list[0]
         indices_k[c * I:(c + 1) * I] = iv[2][0]
@@ -186,18 +187,20 @@ def pack_ell_group(tiles, tile_sizes):
         r_of.append(t.covered_rows - t.tile_i_rng_sel[0])
         c_of.append(np.asarray(iv[2])[t.covered_cols])
         v_of.append(_values(t, ids))
+        i_of.append(np.asarray(ids, np.int64))
     if T:
-        tt, rr, cc, vv = (np.concatenate(x) for x in (t_of, r_of, c_of, v_of))
+        tt, rr, cc, vv, ii = (np.concatenate(x) for x in (t_of, r_of, c_of, v_of, i_of))
         row = tt * I + rr
         order = np.lexsort((cc, row))
-        row, cc, vv = row[order], cc[order], vv[order]
+        row, cc, vv, ii = row[order], cc[order], vv[order], ii[order]
         slot = _rank_in_group(row)
         if len(slot) and slot.max() >= K:
             raise ValueError("an ELL tile row holds more nonzeros than its K")
         a[row, slot] = vv
+        a_ids[row, slot] = ii
         indices_k[row, slot] = cc
     return dict(m_blk=I, nnz_cols=K, num_tiles=T, a=a.reshape(T, I * K), indices_k=indices_k,
-                indices_reordered_i=ir)
+                indices_reordered_i=ir, a_ids=a_ids.reshape(T, I * K))
 
 
 def pack_tc_group(tiles, tile_sizes, params):
@@ -207,6 +210,7 @@ def pack_tc_group(tiles, tile_sizes, params):
     K = tile_sizes[2][1]
     T = len(tiles)
     a = np.zeros((T, I, K), np.float32)
+    a_ids = np.full((T, I, K), -1, np.int32)
     indices = np.empty((T, K), np.int32)
     for c, t in enumerate(tiles):
         op = t.op
@@ -220,6 +224,7 @@ def pack_tc_group(tiles, tile_sizes, params):
             r = t.covered_rows - t.tile_i_rng[0]
             cp = op.k_vals[w].pos(t.covered_cols) - t.tile_k_rng_sel[0]
             a[c, r, cp] = _values(t, t.covered_ids)
+            a_ids[c, r, cp] = t.covered_ids
     poi_is, counts = np.unique([t.tile_pos[0] for t in tiles], return_counts=True)
     indptr = np.concatenate(([0], np.cumsum(counts))).astype(np.int32)
     rows_2d = bool(params['idx_reordered'][0] or params['real_atomic'])
@@ -237,7 +242,8 @@ def pack_tc_group(tiles, tile_sizes, params):
     if k_notsorted:
         indices = np.asarray([int(t.tile_pos[2]) for t in tiles], np.int32)
     return dict(group_size=K, num_windows=len(poi_is), nnzb=T, rows_2d=rows_2d, k_notsorted=k_notsorted,
-                real_atomic=bool(params['real_atomic']), a=a, i_indices=i_ind, indptr=indptr, indices=indices)
+                real_atomic=bool(params['real_atomic']), a=a, i_indices=i_ind, indptr=indptr, indices=indices,
+                a_ids=a_ids)
 
 
 def prepare_inputs_BSPMM(ori_op, op_type, batch_num, formats, out_shape=None, dtype="float32", dev=None):
