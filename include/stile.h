@@ -178,6 +178,15 @@ int stile_csr_sddmm_f32(int32_t M, int32_t N, int32_t nnz, int32_t Kf,
                         const float* X, int64_t ldx, const float* Y,
                         int64_t ldy, float* out, void* stream);
 
+/* Row softmax over a CSR pattern (FP32): for every row i,
+ * out[p] = exp(scale*in[p] - m_i) / sum_q exp(scale*in[q] - m_i) over the
+ * row's nonzeros p, q (m_i = max of scale*in over the row); empty rows write
+ * nothing.  The middle step of sparse attention: SDDMM(Q, K) -> softmax ->
+ * SpMM with V (in the reference only as a mask pattern,
+ * gen_formats_v2.py:2926-3003).  in and out may alias. */
+int stile_csr_row_softmax_f32(int32_t M, const int32_t* indptr, const float* in, float* out, float scale,
+                              void* stream);
+
 /* Build a hybrid SpMM plan from the packed groups (host pointers; copied).
  * The plan regroups every group's products by 16-row output window so one
  * kernel writes each row of C exactly once (no atomics, no zero-fill). */

@@ -249,10 +249,17 @@ def pack_sddmm_tc_group(A, tiles, tile_sizes):
     nnz_id_matrix = ids[np.asarray(iv[0]), :][:, np.asarray(iv[2])]
     nnz_id_matrix.eliminate_zeros()
     K = tile_sizes[2][1]
-    i_indices = np.asarray([np.asarray(iv[0])[t.tile_i_rng[0]:t.tile_i_rng[1] + 1] for t in tiles])
+    # The reference assumes every window and column run is full (M, N
+    # multiples of 16); a run cut short by the matrix edge is padded with its
+    # last index here, which no sampled entry addresses.
+    def full(idx, n):
+        idx = np.asarray(idx)
+        return idx if len(idx) >= n else np.concatenate([idx, np.full(n - len(idx), idx[-1])])
+    i_indices = np.asarray([full(np.asarray(iv[0])[t.tile_i_rng[0]:t.tile_i_rng[1] + 1], tile_sizes[0][1])
+                           for t in tiles])
     max_pos_i = max(t.tile_pos[0] for t in tiles)
     j_lists = [np.asarray(iv[2])[np.asarray(sub_op.k_vals[i])] for i in range(max_pos_i + 1)]
-    j_indices = np.asarray([j_lists[t.tile_pos[0]][t.tile_k_rng[0]:t.tile_k_rng[1] + 1] for t in tiles])
+    j_indices = np.asarray([full(j_lists[t.tile_pos[0]][t.tile_k_rng[0]:t.tile_k_rng[1] + 1], K) for t in tiles])
     per_win = [(nnz_id_matrix[i * 16:(i + 1) * 16, :][:, np.asarray(sub_op.k_vals[i])]).tocsc()
                for i in range(max_pos_i + 1)]
     cscs = [per_win[t.tile_pos[0]][:, t.tile_k_rng[0]:t.tile_k_rng[1] + 1] for t in tiles]

@@ -75,6 +75,8 @@ _SIGNATURES = {
                                            ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                            ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "stile_csr_row_softmax_f32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p]),
     "stile_plan_create_spmm": (ctypes.c_int, [ctypes.POINTER(StileSpmmPack), ctypes.c_int32,
                                               ctypes.POINTER(ctypes.c_void_p)]),
     "stile_plan_create_spmm_batched": (ctypes.c_int, [ctypes.POINTER(StileSpmmPack), ctypes.c_int32,
@@ -197,6 +199,16 @@ def csr_sddmm_f32(indptr, indices, X, Y, out=None):
     _check(load().stile_csr_sddmm_f32(M, Y.shape[0], nnz, Kf, indptr.data_ptr(), indices.data_ptr(),
                                       X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0),
                                       o.data_ptr(), _stream(X)))
+    return o
+
+
+def csr_row_softmax_f32(indptr, vals, scale=1.0, out=None):
+    """Softmax of scale*vals over each CSR row (device float32)."""
+    import torch
+    _dev_check(indptr, vals)
+    o = out if out is not None else torch.empty_like(vals)
+    _check(load().stile_csr_row_softmax_f32(indptr.numel() - 1, indptr.data_ptr(), vals.data_ptr(),
+                                            o.data_ptr(), float(scale), _stream(vals)))
     return o
 
 

@@ -1,0 +1,101 @@
+"""Batched sparse attention over many uneven masks in one graph launch.
+
+For instance b with mask S_b (L_b x L_b), Q_b, K_b, V_b (L_b x d):
+    scores = SDDMM(Q_b, K_b) on S_b           (hybrid SDDMM plan)
+    P      = row_softmax(scores / sqrt(d))    (stile_csr_row_softmax_f32)
+    out_b  = P . V_b                          (hybrid SpMM plan, values = P)
+All instances are one block-diagonal problem: one SDDMM plan and one SpMM
+plan over the stacked rows (stile_plan_create_*_batched), so a forward is
+four kernel launches (SDDMM, softmax, value refresh, SpMM) regardless of the
+number of instances, and the whole forward can be captured into a HIP graph
+(torch.cuda.CUDAGraph) and replayed.
+
+The reference has the masks only as SpMM test patterns (test_LogSparse /
+test_Strided, gen_formats_v2.py:2926-3003) and batches only one pattern over
+many B (my_fuse_formats.py:1296-1428); this pipeline is the batched-attention
+workload built from its SDDMM and SpMM.
+"""
+import math
+import multiprocessing as mp
+import os
+
+import numpy as np
+import scipy.sparse
+import torch
+
+from . import _native, api
+
+
+def _pack_one(args):
+    kind, A, opts = args
+    ones = scipy.sparse.csr_matrix((np.ones(A.nnz, np.float32), A.indices, A.indptr), shape=A.shape)
+    if kind == "spmm":
+        return api.spmm_pack(ones, opts)[0]
+    return api.sddmm_pack(ones, opts)[0]
+
+
+def build_packs(kind, mats, opts=None, workers=None):
+    """Search + pack every matrix, in parallel processes (fork)."""
+    workers = workers or min(len(mats), max(1, (os.cpu_count() or 1)), 16)
+    jobs = [(kind, A, opts) for A in mats]
+    if workers <= 1 or len(mats) == 1:
+        return [_pack_one(j) for j in jobs]
+    ctx = mp.get_context("fork")
+    with ctx.Pool(workers) as pool:
+        return pool.map(_pack_one, jobs, chunksize=1)
+
+
+class BatchedSparseAttention(object):
+    """Sparse attention over a list of CSR masks (one per instance)."""
+
+    def __init__(self, masks, head_dim, blk_dtype="bfloat16", opts=None, device="cuda", workers=None):
+        self.masks = [scipy.sparse.csr_matrix(m) for m in masks]
+        for m in self.masks:
+            m.sort_indices()
+        self.lens = [m.shape[0] for m in self.masks]
+        self.d = head_dim
+        self.device = torch.device(device)
+        self.row_off = np.concatenate([[0], np.cumsum(self.lens)])
+        big = scipy.sparse.block_diag(self.masks, format="csr")
+        big.sort_indices()
+        self.nnz = big.nnz
+        self.indptr = torch.from_numpy(big.indptr.astype(np.int32)).to(self.device)
+        self.indices = big.indices.astype(np.int32)
+        spmm_packs = build_packs("spmm", self.masks, opts, workers)
+        sddmm_packs = build_packs("sddmm", self.masks, opts, workers)
+        self.spmm_plan = _native.Plan.spmm(spmm_packs, blk_dtype)
+        self.sddmm_plan = _native.Plan.sddmm(sddmm_packs, "float32")
+        self.scale = 1.0 / math.sqrt(head_dim)
+        self.scores = torch.empty(self.nnz, device=self.device, dtype=torch.float32)
+        self.probs = torch.empty(self.nnz, device=self.device, dtype=torch.float32)
+        self.graph = None
+
+    def forward(self, Q, K, V, out=None):
+        """Q, K, V: (sum L_b) x d stacked (any float dtype; computed in FP32)."""
+        q = Q.float().contiguous()
+        k = K.float().contiguous()
+        v = V.float().contiguous()
+        self.sddmm_plan.run_sddmm(q, k, out=self.scores)
+        _native.csr_row_softmax_f32(self.indptr, self.scores, self.scale, out=self.probs)
+        self.spmm_plan.set_values(self.probs)
+        return self.spmm_plan.run_spmm(v, out=out)
+
+    def capture(self, Q, K, V):
+        """Capture forward(Q, K, V) into a HIP graph; replay() reruns it on the
+        same input/output buffers (update them in place between replays)."""
+        self.gQ, self.gK, self.gV = Q, K, V
+        self.gout = torch.empty((Q.shape[0], self.d), device=self.device, dtype=torch.float32)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(2):
+                self.forward(Q, K, V, out=self.gout)
+        torch.cuda.current_stream().wait_stream(s)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.forward(self.gQ, self.gK, self.gV, out=self.gout)
+        return self.gout
+
+    def replay(self):
+        self.graph.replay()
+        return self.gout

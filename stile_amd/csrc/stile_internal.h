@@ -90,6 +90,9 @@ hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,
 hipError_t launch_set_values(const SpmmPlanDev& p, const float* vals, const int32_t* tc_ids, int64_t n_tc,
                              const int32_t* res_ids, int64_t n_res, hipStream_t s);
 
+hipError_t launch_row_softmax(int M, const int32_t* indptr, const float* in, float* out, float scale,
+                              hipStream_t s);
+
 // Zero-fill kernels (used instead of hipMemsetAsync so that every *_run call
 // records only kernel nodes under HIP graph capture).
 hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t s);

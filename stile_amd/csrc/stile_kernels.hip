@@ -897,6 +897,38 @@ hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,
 }
 
 // ---------------------------------------------------------------------------
+// Row softmax over CSR rows: one wave per row; max and sum reduced with xor
+// shuffles across the 64 lanes.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void row_softmax_kernel(int M, const int32_t* __restrict__ indptr,
+                                                          const float* in, float* out, float scale) {
+  const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (row >= M) return;
+  const int lane = threadIdx.x & 63;
+  const int p0 = indptr[row], p1 = indptr[row + 1];
+  if (p0 >= p1) return;
+  float m = -INFINITY;
+  for (int p = p0 + lane; p < p1; p += 64) m = fmaxf(m, scale * in[p]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  float s = 0.f;
+  for (int p = p0 + lane; p < p1; p += 64) s += __expf(scale * in[p] - m);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  const float inv = 1.f / s;
+  for (int p = p0 + lane; p < p1; p += 64) out[p] = __expf(scale * in[p] - m) * inv;
+}
+
+hipError_t launch_row_softmax(int M, const int32_t* indptr, const float* in, float* out, float scale,
+                              hipStream_t s) {
+  if (M == 0) return hipSuccess;
+  const int64_t waves = M;
+  hipLaunchKernelGGL(row_softmax_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, M, indptr, in, out,
+                     scale);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Value refresh.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void set_values_kernel(const float* __restrict__ vals,

@@ -748,6 +748,15 @@ int stile_csr_sddmm_f32(int32_t M, int32_t N, int32_t nnz, int32_t Kf, const int
   return e == hipSuccess ? STILE_OK : hip_fail(e, "csr_sddmm launch");
 }
 
+int stile_csr_row_softmax_f32(int32_t M, const int32_t* indptr, const float* in, float* out, float scale,
+                              void* stream) {
+  if (M < 0) return fail(STILE_E_INVALID, "negative M");
+  if (M == 0) return STILE_OK;
+  if (!indptr || !in || !out) return fail(STILE_E_INVALID, "null pointer");
+  hipError_t e = stile::launch_row_softmax(M, indptr, in, out, scale, (hipStream_t)stream);
+  return e == hipSuccess ? STILE_OK : hip_fail(e, "row_softmax launch");
+}
+
 int stile_plan_create_spmm(const StileSpmmPack* pack, int32_t blk_dtype, StilePlan** out) {
   return create_spmm(pack, 1, blk_dtype, out);
 }

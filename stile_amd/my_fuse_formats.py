@@ -266,6 +266,13 @@ def prepare_inputs_BSPMM(ori_op, op_type, batch_num, formats, out_shape=None, dt
     return dict(M=int(ori_op.idx_lens[0]), K=int(ori_op.idx_lens[2]), ell=ell, tc=tc, order=order)
 
 
+def _pad_run(idx, n):
+    idx = np.asarray(idx)
+    if len(idx) >= n:
+        return idx[:n]
+    return np.concatenate([idx, np.full(n - len(idx), idx[-1], idx.dtype)])
+
+
 def prepare_inputs_BSDDMM(ori_op, op_type, batch_num, formats, out_shape=None, dtype="float32", dev=None):
     """1D groups: per-nonzero (mid_idx, j_idx, remap) padded to a multiple of
     ty*group_size by repeating the last entry; TC groups: block rows/cols,
@@ -289,9 +296,12 @@ def prepare_inputs_BSDDMM(ori_op, op_type, batch_num, formats, out_shape=None, d
             ts = fmt[1]
             op = tiles[0].op
             iv = op.idx_values_list[0]
-            i_ind = np.asarray([np.asarray(iv[0])[t.tile_i_rng[0]:t.tile_i_rng[1] + 1] for t in tiles], np.int32)
-            j_ind = np.asarray([np.asarray(iv[2])[op.k_vals[t.tile_pos[0]][t.tile_k_rng[0]:t.tile_k_rng[1] + 1]]
+            # a window or column run cut short by the matrix edge is padded
+            # with its last index: no sampled entry points into the padding
+            i_ind = np.asarray([_pad_run(np.asarray(iv[0])[t.tile_i_rng[0]:t.tile_i_rng[1] + 1], ts[0][1])
                                 for t in tiles], np.int32)
+            j_ind = np.asarray([_pad_run(np.asarray(iv[2])[op.k_vals[t.tile_pos[0]][t.tile_k_rng[0]:t.tile_k_rng[1] + 1]],
+                                         ts[2][1]) for t in tiles], np.int32)
             dense, outs, cnt = [], [], [0]
             for t in tiles:
                 e = t.entries()

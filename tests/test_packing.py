@@ -89,9 +89,13 @@ def test_spmm_pack_matches_oracle_and_reexpands(name, A, opts):
     assert abs(diff).max() == 0 if diff.nnz else True
 
 
-@pytest.mark.parametrize("scale,Kf", [(9, 32), (10, 64)])
+@pytest.mark.parametrize("scale,Kf", [(9, 32), (10, 64), ("mask891", 64)])
 def test_sddmm_pack_matches_oracle(scale, Kf):
-    A = datasets.rmat_graph(scale, seed=scale)
+    if scale == "mask891":
+        # 891 rows: the last 16-row window is cut short by the matrix edge
+        A = datasets.attention_mask(891, band=32, n_random=4, rng=np.random.default_rng(0))
+    else:
+        A = datasets.rmat_graph(scale, seed=scale)
     pack, tiles, op = api.sddmm_pack(A, Kf=Kf)
     formats = my_fuse_formats.get_formats_from_selected_tiles(tiles, ['A'], 16)
     seen = np.zeros(A.nnz, bool)
