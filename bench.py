@@ -1,16 +1,29 @@
-"""Benchmark: hybrid fused SpMM on one MI355X (pruned-layer workload).
+"""Benchmark of the hybrid sparse-tile SpMM / SDDMM path on one MI355X.
 
-Workload (one step): C = A . B with A the 3072 x 768 block-pruned layer
-(10 % of the 16x16 blocks dense + 1 % unstructured residual, 257,265
-nonzeros, seed 0), B 768 x 4096 FP32 resident in HBM, C 3072 x 4096 FP32.
-The tile set comes from the STile search (greedy + withdraw, cost model);
-dense blocks run BF16 MFMA with FP32 accumulation, the rest FP32.
+Default workload (one step = one launch of the fused SpMM):
+  C = A . B, A the 3072 x 768 block-pruned layer (10 % of the 16x16 blocks
+  dense + 1 % unstructured residual, 257,265 nonzeros, seed 0), B 768 x 4096
+  FP32 resident in HBM, C 3072 x 4096 FP32.  Tiles from the STile search
+  (greedy + withdraw, cost model); dense blocks run BF16 MFMA with FP32
+  accumulation, the sparse rest FP32.
 
-Printed: one JSON line with GFLOP/s (2 nnz N per step), the HBM roofline of
-the fused kernel (algorithmic bytes nnz*8 + 4(M+1) + K*N*4 + M*N*4 per
-launch over the HIP-event-timed launch duration), the plain FP32 CSR HIP
-kernel on the same A and N, and the reference's CPU path (scipy CSR x dense,
-FP32, one core) timed on a bounded sample.
+Other workloads (--workload), each printed as its own JSON line:
+  uniform_spmm  A 4096^2, 167,772 uniform nonzeros U(-1,1), B 4096 x 64 FP32
+  rmat_sddmm    R-MAT scale 16 (+ self loops), X, Y 65536 x 32 FP32
+  gcn           GCN layer on R-MAT scale 20 (32.45M nonzeros, symmetric
+                normalised), H 1M x 128, W 128 x 128 FP32: forward A.(HW),
+                backward A^T.dY and SDDMM dA (one step = forward + backward)
+  attention     64 sparse-attention instances, L ~ U[512, 8192], d = 64,
+                band 128 + 16 random columns: SDDMM(QK^T), row softmax,
+                SpMM with V, replayed as one HIP graph (one step = one replay)
+  all           every workload above, one line each
+
+Each line carries GFLOP/s (2 nnz N per sparse product), the HBM roofline of
+the dominant kernel (algorithmic bytes over its HIP-event-timed launch
+duration, on the stream it is launched on), the plain CSR HIP kernel and the
+vendor CSR SpMM (torch.sparse, rocSPARSE) on the same operands, the error
+against the FP64 oracle on a sample, and a CPU baseline timed on a bounded
+sample of the same workload.
 """
 import argparse
 import json
@@ -24,23 +37,31 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def _alg_bytes(nnz, M, K, N):
-    return nnz * 8 + 4 * (M + 1) + K * N * 4 + M * N * 4
+def spmm_alg_bytes(nnz, M, K, N, s_b=4):
+    """nnz (value + index) + indptr + B once + C once."""
+    return nnz * 8 + 4 * (M + 1) + K * N * s_b + M * N * 4
 
 
-def _measured_traffic(N):
-    """HBM bytes per launch of the fused kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration),
-    for the N it was measured at; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "hyb_spmm_traffic_pruned_layer.json")
-    if N != 4096 or not os.path.exists(path):
+def sddmm_alg_bytes(nnz, M, Ncols, K):
+    """index + output per nonzero, indptr, X and Y once."""
+    return nnz * 8 + 4 * (M + 1) + (M + Ncols) * K * 4
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration factors stored with it), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
     return int((t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024)
 
 
-def _time_launches(fn, steps, warmup):
+def time_launches(fn, steps, warmup):
+    """ms per call of fn, HIP events on torch's current stream (the stream
+    every entry point here launches on)."""
     import torch
     for _ in range(warmup):
         fn()
@@ -52,54 +73,97 @@ def _time_launches(fn, steps, warmup):
         fn()
     e.record()
     torch.cuda.synchronize()
-    return s.elapsed_time(e) / steps  # ms per launch
+    return s.elapsed_time(e) / steps
 
 
-def cpu_baseline(A, B, seconds=10.0):
-    """The reference's CPU path (scipy csr.astype(float32).dot(B), one core)
-    on a bounded sample: the first 512 columns of B, repeated for ~seconds."""
-    import numpy as np
-    from oracle import ops
-    Bs = np.ascontiguousarray(B[:, :512])
-    ops.spmm_ref_fp32(A, Bs)
+def roofline(alg_bytes, ms, traffic=None):
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "alg_bytes": int(alg_bytes)}
+
+
+def line(metric, value, unit, ms, args, dtype, config, **extra):
+    out = {"metric": metric, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def _repeat_for(fn, seconds, max_runs=200):
+    fn()
     t0 = time.perf_counter()
     n = 0
     while True:
-        ops.spmm_ref_fp32(A, Bs)
+        fn()
         n += 1
-        if time.perf_counter() - t0 >= seconds or n >= 200:
+        if time.perf_counter() - t0 >= seconds or n >= max_runs:
             break
-    dt = (time.perf_counter() - t0) / n
-    return {"value": round(2 * A.nnz * Bs.shape[1] / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
-            "kind": "port",
-            "sample": f"scipy CSR(3072x768, nnz={A.nnz}) . B[:, :512] FP32, {n} runs of {dt * 1e3:.1f} ms"}
+    return (time.perf_counter() - t0) / n, n
 
 
-def check_error(A, B, C_gpu, cols=256):
-    """Max norm-wise relative error vs FP64 on the first `cols` columns."""
+def cpu_spmm_baseline(A, B, cols, seconds, label):
+    """The reference's CPU path (scipy csr.astype(float32).dot(B), one core)
+    on a bounded sample: the first `cols` columns of B."""
     import numpy as np
     from oracle import ops
-    ref = ops.spmm_fp64(A, B[:, :cols])
-    return ops.max_rel_err(C_gpu[:, :cols], ref)
+    Bs = np.ascontiguousarray(B[:, :cols])
+    dt, n = _repeat_for(lambda: ops.spmm_ref_fp32(A, Bs), seconds)
+    return {"value": round(2 * A.nnz * Bs.shape[1] / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+            "kind": "port", "sample": f"scipy CSR({label}, nnz={A.nnz}) . B[:, :{Bs.shape[1]}] FP32, "
+                                      f"{n} runs of {dt * 1e3:.1f} ms"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the tile path shards by "
-              "output rows, see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def vendor_csr_spmm(A, Bd):
+    """torch.sparse CSR x dense (rocSPARSE) on the same A; None if unavailable."""
     import numpy as np
     import torch
-    from stile_amd import _native, api, datasets
+    try:
+        At = torch.sparse_csr_tensor(torch.from_numpy(A.indptr.astype(np.int64)),
+                                     torch.from_numpy(A.indices.astype(np.int64)),
+                                     torch.from_numpy(A.data.astype(np.float32)), size=A.shape).to(Bd.device)
+        return lambda: torch.sparse.mm(At, Bd)
+    except Exception:                                            # noqa: BLE001
+        return None
 
-    dev = torch.device("cuda:0")
+
+def _csr_dev(A, dev):
+    import numpy as np
+    import torch
+    return (torch.from_numpy(A.indptr.astype(np.int32)).to(dev),
+            torch.from_numpy(A.indices.astype(np.int32)).to(dev),
+            torch.from_numpy(A.data.astype(np.float32)).to(dev))
+
+
+def _spmm_comparators(A, Bd, steps, warmup):
+    import torch
+    from stile_amd import _native
+    ip, ix, vv = _csr_dev(A, Bd.device)
+    C2 = torch.empty((A.shape[0], Bd.shape[1]), device=Bd.device, dtype=torch.float32)
+    ms_csr = time_launches(lambda: _native.csr_spmm_f32(ip, ix, vv, Bd, C2), steps, warmup)
+    vfn = vendor_csr_spmm(A, Bd)
+    ms_vendor = None
+    if vfn is not None:
+        try:
+            ms_vendor = time_launches(vfn, steps, warmup)
+        except Exception:                                        # noqa: BLE001
+            ms_vendor = None
+    return ms_csr, ms_vendor
+
+
+def _gf(flops, ms):
+    return None if ms is None else round(flops / (ms * 1e-3) / 1e9, 2)
+
+
+# ---------------------------------------------------------------------------
+# Workloads.
+# ---------------------------------------------------------------------------
+
+def run_pruned_layer(args, dev):
+    import numpy as np
+    import torch
+    from oracle import ops
+    from stile_amd import api, datasets
     N = args.n
     A, B = datasets.pruned_layer(seed=0, N=N)
     M, K = A.shape
@@ -109,50 +173,264 @@ def main():
     info = plan.info()
     Bd = torch.from_numpy(B).to(dev)
     C = torch.empty((M, N), device=dev, dtype=torch.float32)
-    ms = _time_launches(lambda: plan.run_spmm(Bd, C), args.steps, args.warmup)
+    ms = time_launches(lambda: plan.run_spmm(Bd, C), args.steps, args.warmup)
     C_h = C.cpu().numpy()
-
-    ip = torch.from_numpy(A.indptr.astype(np.int32)).to(dev)
-    ix = torch.from_numpy(A.indices.astype(np.int32)).to(dev)
-    vv = torch.from_numpy(A.data.astype(np.float32)).to(dev)
-    C2 = torch.empty((M, N), device=dev, dtype=torch.float32)
-    ms_csr = _time_launches(lambda: _native.csr_spmm_f32(ip, ix, vv, Bd, C2), max(args.steps // 4, 10), 5)
-
+    ms_csr, ms_vendor = _spmm_comparators(A, Bd, max(args.steps // 4, 10), 5)
     flops = 2 * A.nnz * N
-    gflops = flops / (ms * 1e-3) / 1e9
-    gflops_csr = flops / (ms_csr * 1e-3) / 1e9
-    byt = _alg_bytes(A.nnz, M, K, N)
-    achieved = byt / (ms * 1e-3) / 1e9
-
-    # secondary: the north-star width N = 128
     B128 = torch.from_numpy(np.ascontiguousarray(B[:, :128])).to(dev)
     C128 = torch.empty((M, 128), device=dev, dtype=torch.float32)
-    ms128 = _time_launches(lambda: plan.run_spmm(B128, C128), args.steps, args.warmup)
-    ms128_csr = _time_launches(lambda: _native.csr_spmm_f32(ip, ix, vv, B128, C128), args.steps, args.warmup)
-
-    out = {
-        "metric": "SpMM GFLOP/s (2*nnz*N), hybrid fused vs plain CSR",
-        "value": round(gflops, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "bf16 MFMA blocks + f32 sparse, f32 accumulate", "data": "synthetic",
-        "config": {"workload": "hybrid SpMM, block-pruned layer A 3072x768 (10% of 16x16 blocks + 1% residual, "
-                               f"nnz={A.nnz}) x B 768x{N} FP32", "M": M, "K": K, "N": N, "nnz": int(A.nnz),
-                   "global_batch": 1, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _measured_traffic(N),
-                     "alg_bytes": int(byt)},
-        "plain_csr_gflops": round(gflops_csr, 2), "hybrid_over_csr": round(gflops / gflops_csr, 3),
-        "n128": {"hybrid_us": round(ms128 * 1e3, 2), "csr_us": round(ms128_csr * 1e3, 2),
-                 "hybrid_gflops": round(2 * A.nnz * 128 / (ms128 * 1e-3) / 1e9, 2),
-                 "hbm_frac": round(_alg_bytes(A.nnz, M, K, 128) / (ms128 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "plan": {"windows": info["num_windows"], "mfma_chunks": info["num_chunks"],
-                 "block_nnz": info["block_nnz"], "sparse_nnz": info["sparse_nnz"],
-                 "search_and_plan_s": round(t_plan, 3)},
-    }
+    ms128 = time_launches(lambda: plan.run_spmm(B128, C128), args.steps, args.warmup)
+    ms128_csr, ms128_vendor = _spmm_comparators(A, B128, args.steps, args.warmup)
+    traffic = measured_traffic("hyb_spmm_traffic_pruned_layer.json") if N == 4096 else None
+    out = line("SpMM GFLOP/s (2*nnz*N), hybrid fused vs plain CSR", flops / (ms * 1e-3) / 1e9, "GFLOP/s",
+               ms, args, "bf16 MFMA blocks + f32 sparse, f32 accumulate",
+               {"workload": "hybrid SpMM, block-pruned layer A 3072x768 (10% of 16x16 blocks + 1% residual, "
+                            f"nnz={A.nnz}) x B 768x{N} FP32", "M": M, "K": K, "N": N, "nnz": int(A.nnz),
+                "global_batch": 1, "parallelism": "single GPU"},
+               roofline=roofline(spmm_alg_bytes(A.nnz, M, K, N), ms, traffic),
+               plain_csr_gflops=_gf(flops, ms_csr), vendor_csr_gflops=_gf(flops, ms_vendor),
+               hybrid_over_csr=round(ms_csr / ms, 3),
+               hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),
+               n128={"hybrid_us": round(ms128 * 1e3, 2), "csr_us": round(ms128_csr * 1e3, 2),
+                     "vendor_us": None if ms128_vendor is None else round(ms128_vendor * 1e3, 2),
+                     "hybrid_gflops": _gf(2 * A.nnz * 128, ms128),
+                     "hbm_frac": roofline(spmm_alg_bytes(A.nnz, M, K, 128), ms128)["frac"]},
+               plan={"windows": info["num_windows"], "mfma_chunks": info["num_chunks"],
+                     "block_nnz": info["block_nnz"], "sparse_nnz": info["sparse_nnz"],
+                     "search_and_plan_s": round(t_plan, 3)})
     if not args.no_cpu:
-        out["max_rel_err_vs_fp64"] = check_error(A, B, C_h)
-        out["cpu_baseline"] = cpu_baseline(A, B)
-    print(json.dumps(out), flush=True)
+        out["max_rel_err_vs_fp64"] = ops.max_rel_err(C_h[:, :256], ops.spmm_fp64(A, B[:, :256]))
+        out["cpu_baseline"] = cpu_spmm_baseline(A, B, 512, 10.0, "3072x768")
+    return out
+
+
+def run_uniform_spmm(args, dev):
+    import torch
+    from oracle import ops
+    from stile_amd import api, datasets
+    A, B = datasets.uniform_spmm(seed=0)
+    M, K = A.shape
+    N = B.shape[1]
+    t0 = time.perf_counter()
+    plan = api.build_spmm_plan(A, blk_dtype="float32", N=N)
+    t_plan = time.perf_counter() - t0
+    Bd = torch.from_numpy(B).to(dev)
+    C = torch.empty((M, N), device=dev, dtype=torch.float32)
+    ms = time_launches(lambda: plan.run_spmm(Bd, C), args.steps, args.warmup)
+    ms_csr, ms_vendor = _spmm_comparators(A, Bd, args.steps, args.warmup)
+    flops = 2 * A.nnz * N
+    out = line("SpMM GFLOP/s (2*nnz*N), uniform 1% A", flops / (ms * 1e-3) / 1e9, "GFLOP/s", ms, args, "f32",
+               {"workload": f"SpMM FP32, uniform A 4096x4096 nnz={A.nnz} x B 4096x{N}", "M": M, "K": K, "N": N,
+                "nnz": int(A.nnz), "parallelism": "single GPU"},
+               roofline=roofline(spmm_alg_bytes(A.nnz, M, K, N), ms),
+               plain_csr_gflops=_gf(flops, ms_csr), vendor_csr_gflops=_gf(flops, ms_vendor),
+               hybrid_over_csr=round(ms_csr / ms, 3),
+               hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),
+               plan=dict(plan.info(), search_and_plan_s=round(t_plan, 3)))
+    if not args.no_cpu:
+        out["max_rel_err_vs_fp64"] = ops.max_rel_err(C.cpu().numpy(), ops.spmm_fp64(A, B))
+        out["cpu_baseline"] = cpu_spmm_baseline(A, B, N, 10.0, "4096x4096")
+    return out
+
+
+def run_rmat_sddmm(args, dev):
+    import numpy as np
+    import torch
+    from oracle import ops
+    from stile_amd import _native, api, datasets
+    A, X, Y = datasets.sddmm_rmat(seed=0)
+    M, Nc = A.shape
+    Kf = X.shape[1]
+    t0 = time.perf_counter()
+    plan = api.build_sddmm_plan(A, Kf=Kf)
+    t_plan = time.perf_counter() - t0
+    Xd, Yd = torch.from_numpy(X).to(dev), torch.from_numpy(Y).to(dev)
+    out_d = torch.empty(A.nnz, device=dev, dtype=torch.float32)
+    ms = time_launches(lambda: plan.run_sddmm(Xd, Yd, out_d), args.steps, args.warmup)
+    ip, ix, _ = _csr_dev(A, dev)
+    o2 = torch.empty_like(out_d)
+    ms_csr = time_launches(lambda: _native.csr_sddmm_f32(ip, ix, Xd, Yd, o2), args.steps, args.warmup)
+    flops = 2 * A.nnz * Kf
+    out = line("SDDMM GFLOP/s (2*nnz*K)", flops / (ms * 1e-3) / 1e9, "GFLOP/s", ms, args, "f32",
+               {"workload": f"SDDMM FP32, R-MAT scale 16 + self loops (nnz={A.nnz}), X, Y {M}x{Kf}",
+                "M": M, "N": Nc, "K": Kf, "nnz": int(A.nnz), "parallelism": "single GPU"},
+               roofline=roofline(sddmm_alg_bytes(A.nnz, M, Nc, Kf), ms),
+               plain_csr_gflops=_gf(flops, ms_csr), hybrid_over_csr=round(ms_csr / ms, 3),
+               plan=dict(plan.info(), search_and_plan_s=round(t_plan, 3)))
+    if not args.no_cpu:
+        got = out_d.cpu().numpy()
+        ref = ops.sddmm_fp64(A, X, Y)
+        out["max_rel_err_vs_fp64"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        rows = np.repeat(np.arange(M), np.diff(A.indptr))
+        s = min(A.nnz, 1 << 18)
+
+        def cpu():
+            return (X[rows[:s]] * Y[A.indices[:s]]).sum(1)
+        dt, n = _repeat_for(cpu, 10.0)
+        out["cpu_baseline"] = {"value": round(2 * s * Kf / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"numpy (X[row]*Y[col]).sum(1) over the first {s} nonzeros, "
+                                         f"{n} runs of {dt * 1e3:.1f} ms"}
+    return out
+
+
+def run_gcn(args, dev):
+    import numpy as np
+    import torch
+    from stile_amd import _native, datasets
+    from stile_amd.autograd import SparseOperand
+    A, H, W = datasets.gcn_graph(args.gcn_scale)
+    M = A.shape[0]
+    F = H.shape[1]
+    t0 = time.perf_counter()
+    op = SparseOperand(A, blk_dtype="float32", device=dev, N=F)
+    t_plan = time.perf_counter() - t0
+    Hd = torch.from_numpy(H).to(dev).requires_grad_(True)
+    Wd = torch.from_numpy(W).to(dev).requires_grad_(True)
+    vals = op.values.clone().requires_grad_(True)
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    dY = torch.randn((M, F), device=dev, generator=g)
+
+    def step():
+        Hd.grad = None
+        Wd.grad = None
+        vals.grad = None
+        y = op.spmm(Hd @ Wd, vals)
+        y.backward(dY)
+    steps = max(args.steps // 20, 5)
+    ms = time_launches(step, steps, 2)
+    HW = (Hd @ Wd).detach().contiguous()
+    C = torch.empty_like(HW)
+    ms_fwd = time_launches(lambda: op.plan.run_spmm(HW, C), steps, 2)
+    ms_t = time_launches(lambda: op.plan_t.run_spmm(dY, C), steps, 2)
+    sd = torch.empty(A.nnz, device=dev)
+    ms_sd = time_launches(lambda: op.sddmm_plan.run_sddmm(dY, HW, sd), steps, 2)
+    ms_csr, ms_vendor = _spmm_comparators(A, HW, steps, 2)
+    ip, ix, _ = _csr_dev(A, dev)
+    ms_csr_sd = time_launches(lambda: _native.csr_sddmm_f32(ip, ix, dY, HW, sd), steps, 2)
+    flops = 3 * 2 * A.nnz * F
+    out = line("GCN layer sparse GFLOP/s (A.(HW), A^T.dY, SDDMM dA: 3 x 2*nnz*F per fwd+bwd step)",
+               flops / (ms * 1e-3) / 1e9, "GFLOP/s", ms, args, "f32",
+               {"workload": f"GCN layer fwd+bwd, R-MAT scale {args.gcn_scale} symmetric normalised "
+                            f"(nnz={A.nnz}), H {M}x{F}, W {F}x{F}", "M": M, "F": F, "nnz": int(A.nnz),
+                "parallelism": "single GPU"},
+               roofline=roofline(spmm_alg_bytes(A.nnz, M, M, F), ms_fwd),
+               kernels_ms={"spmm_fwd": round(ms_fwd, 4), "spmm_transposed": round(ms_t, 4),
+                           "sddmm": round(ms_sd, 4), "plain_csr_spmm": round(ms_csr, 4),
+                           "vendor_csr_spmm": None if ms_vendor is None else round(ms_vendor, 4),
+                           "plain_csr_sddmm": round(ms_csr_sd, 4)},
+               plan={"spmm": op.plan.info(), "sddmm": op.sddmm_plan.info(), "symmetric_reuse": op.symmetric,
+                     "search_and_plan_s": round(t_plan, 3)})
+    if not args.no_cpu:
+        rng = np.random.default_rng(0)
+        rows = rng.choice(M, 2000, replace=False)
+        hw = HW.double().cpu().numpy()
+        ref = A[rows].astype(np.float64) @ hw
+        op.plan.run_spmm(HW, C)
+        got = C[torch.from_numpy(rows).to(dev)].double().cpu().numpy()
+        out["max_rel_err_vs_fp64_sample"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        HWh = np.ascontiguousarray(HW.cpu().numpy())
+        out["cpu_baseline"] = cpu_spmm_baseline(A, HWh, 16, 10.0, f"{M}x{M}")
+    return out
+
+
+def run_attention(args, dev):
+    import numpy as np
+    import torch
+    from stile_amd import datasets
+    from stile_amd.attention import BatchedSparseAttention
+    masks = datasets.attention_masks(count=64)
+    d = 64
+    t0 = time.perf_counter()
+    att = BatchedSparseAttention(masks, d, blk_dtype="bfloat16", device=dev)
+    t_plan = time.perf_counter() - t0
+    n = int(att.row_off[-1])
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    Q, K, V = (torch.randn(n, d, device=dev, generator=g).to(torch.bfloat16) for _ in range(3))
+    eager = att.forward(Q, K, V).clone()
+    att.capture(Q, K, V)
+    ms = time_launches(att.replay, max(args.steps // 4, 10), 3)
+    q, k, v = Q.float(), K.float(), V.float()
+    ms_sd = time_launches(lambda: att.sddmm_plan.run_sddmm(q, k, att.scores), 20, 3)
+    ms_sp = time_launches(lambda: att.spmm_plan.run_spmm(v, att.gout), 20, 3)
+    flops = 2 * 2 * att.nnz * d
+    out = line("batched sparse attention GFLOP/s (SDDMM + SpMM, 2 x 2*nnz*d per graph replay)",
+               flops / (ms * 1e-3) / 1e9, "GFLOP/s", ms, args, "bf16 inputs, f32 scores/softmax, bf16 MFMA blocks",
+               {"workload": f"64 attention instances, L~U[512,8192] (sum {n}), d={d}, band 128 + 16 random "
+                            f"(nnz={att.nnz}), one HIP graph", "instances": 64, "rows": n, "nnz": int(att.nnz),
+                "parallelism": "single GPU"},
+               roofline=roofline(spmm_alg_bytes(att.nnz, n, n, d), ms_sp),
+               kernels_ms={"sddmm": round(ms_sd, 4), "spmm": round(ms_sp, 4)},
+               plan={"spmm": att.spmm_plan.info(), "sddmm": att.sddmm_plan.info(),
+                     "search_and_plan_s": round(t_plan, 3)})
+    if not args.no_cpu:
+        att.replay()
+        torch.cuda.synchronize()
+        errs = []
+        for b in (0, 31, 63):
+            m = masks[b]
+            a, e = att.row_off[b], att.row_off[b + 1]
+            qq, kk, vv = (x[a:e].double().cpu().numpy() for x in (Q, K, V))
+            rows = np.repeat(np.arange(m.shape[0]), np.diff(m.indptr))
+            s = (qq[rows] * kk[m.indices]).sum(1) / np.sqrt(d)
+            mx = np.maximum.reduceat(s, m.indptr[:-1])
+            w = np.exp(s - mx[rows])
+            w /= np.add.reduceat(w, m.indptr[:-1])[rows]
+            ref = np.zeros_like(qq)
+            np.add.at(ref, rows, w[:, None] * vv[m.indices])
+            got = att.gout[a:e].double().cpu().numpy()
+            errs.append(float(np.abs(got - ref).max() / np.abs(ref).max()))
+        out["max_rel_err_vs_fp64_3_instances"] = max(errs)
+        out["graph_replay_equals_eager"] = bool(torch.equal(att.gout, eager))
+        m = masks[0]
+        qq = Q[:m.shape[0]].float().cpu().numpy()
+        kk = K[:m.shape[0]].float().cpu().numpy()
+        vv = V[:m.shape[0]].float().cpu().numpy()
+        rows = np.repeat(np.arange(m.shape[0]), np.diff(m.indptr))
+
+        def cpu():
+            s = (qq[rows] * kk[m.indices]).sum(1) * np.float32(1 / np.sqrt(d))
+            mx = np.maximum.reduceat(s, m.indptr[:-1])
+            w = np.exp(s - mx[rows])
+            w /= np.add.reduceat(w, m.indptr[:-1])[rows]
+            P = m.copy()
+            P.data = w.astype(np.float32)
+            return P @ vv
+        dt, nrun = _repeat_for(cpu, 10.0)
+        out["cpu_baseline"] = {"value": round(2 * 2 * m.nnz * d / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"numpy/scipy SDDMM + softmax + SpMM on instance 0 (L={m.shape[0]}, "
+                                         f"nnz={m.nnz}), {nrun} runs of {dt * 1e3:.1f} ms"}
+    return out
+
+
+WORKLOADS = {"pruned_layer": run_pruned_layer, "uniform_spmm": run_uniform_spmm,
+             "rmat_sddmm": run_rmat_sddmm, "gcn": run_gcn, "attention": run_attention}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=4096, help="dense width of the pruned-layer workload")
+    ap.add_argument("--workload", default="pruned_layer", choices=sorted(WORKLOADS) + ["all"])
+    ap.add_argument("--gcn-scale", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the tile path would shard "
+              "by output rows with B replicated; only the one-GPU share is built)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    dev = torch.device("cuda:0")
+    names = [args.workload]
+    if args.workload == "all":
+        names = ["pruned_layer", "uniform_spmm", "rmat_sddmm", "attention", "gcn"]
+    for name in names:
+        print(json.dumps(WORKLOADS[name](args, dev)), flush=True)
 
 
 if __name__ == "__main__":

@@ -99,6 +99,125 @@ def sddmm_pack(A, opts=None, Kf=32):
     return my_fuse_formats.prepare_inputs_BSDDMM(op, 'sddmm', 1, formats), tiles, op
 
 
+# ---------------------------------------------------------------------------
+# Row-partitioned search for large matrices.  The search is a greedy pass over
+# 16-row windows and row buckets; cutting A into row blocks of whole windows,
+# searching each block in its own process and stacking the packs gives one
+# plan for A.  ELL tiles then group rows of similar length within a block
+# rather than across the whole matrix (blocks hold ~10^5 rows).
+# ---------------------------------------------------------------------------
+
+def parallel_map(fn, jobs, workers=None):
+    """fn over jobs in worker processes: forked while this process has not
+    touched the GPU, spawned otherwise (a forked copy of an initialised HIP
+    runtime is not usable)."""
+    import multiprocessing as mp
+    import os
+    workers = min(len(jobs), workers or min(os.cpu_count() or 1, 16))
+    if workers <= 1:
+        return [fn(j) for j in jobs]
+    try:
+        import torch
+        spawn = torch.cuda.is_initialized()
+    except ImportError:
+        spawn = False
+    ctx = mp.get_context("spawn" if spawn else "fork")
+    with ctx.Pool(workers) as pool:
+        return pool.map(fn, jobs, chunksize=1)
+
+
+def row_blocks(A, parts):
+    """Row cut points (multiples of 16) giving `parts` blocks of about equal nnz."""
+    A = _csr_of(A)
+    M = A.shape[0]
+    if parts <= 1 or M <= 16:
+        return np.array([0, M])
+    targets = np.linspace(0, A.nnz, parts + 1)[1:-1]
+    cuts = np.searchsorted(A.indptr, targets)
+    cuts = np.unique(np.clip((cuts // 16) * 16, 16, (M - 1) // 16 * 16))
+    return np.concatenate([[0], cuts[cuts > 0], [M]]).astype(np.int64)
+
+
+def _block_job(args):
+    kind, A, opts, width = args
+    if kind == "spmm":
+        return spmm_pack(A, opts, width)[0]
+    return sddmm_pack(A, opts, width)[0]
+
+
+def _shift_ids(a, off):
+    a = np.array(a, np.int32, copy=True)
+    a[a >= 0] += off
+    return a
+
+
+def stack_spmm_packs(packs, cuts, nnz_offs, M, K):
+    """One SpMM pack for A from the packs of its row blocks A[cuts[b]:cuts[b+1]]
+    (rows shifted, the block sentinel row -> M, value ids shifted by the
+    block's first CSR position)."""
+    ell, tc = [], []
+    for pk, r0, r1, z0 in zip(packs, cuts[:-1], cuts[1:], nnz_offs):
+        Mb = int(r1 - r0)
+        for g in pk["ell"]:
+            g = dict(g)
+            rows = np.asarray(g["indices_reordered_i"], np.int64)
+            g["indices_reordered_i"] = np.where(rows == Mb, M, rows + r0).astype(np.int32)
+            if g.get("a_ids") is not None:
+                g["a_ids"] = _shift_ids(g["a_ids"], z0)
+            ell.append(g)
+        for g in pk["tc"]:
+            g = dict(g)
+            ii = np.asarray(g["i_indices"], np.int64)
+            if g["rows_2d"]:
+                g["i_indices"] = np.where(ii == Mb, M, ii + r0).astype(np.int32)
+            else:
+                g["i_indices"] = (ii + r0 // 16).astype(np.int32)
+            if g.get("a_ids") is not None:
+                g["a_ids"] = _shift_ids(g["a_ids"], z0)
+            tc.append(g)
+    return dict(M=int(M), K=int(K), ell=ell, tc=tc)
+
+
+def stack_sddmm_packs(packs, cuts, nnz_offs, M, N, nnz):
+    """One SDDMM pack for A from the packs of its row blocks."""
+    g1d, tc = [], []
+    for pk, r0, z0 in zip(packs, cuts[:-1], nnz_offs):
+        for g in pk["g1d"]:
+            g1d.append(dict(mid_idx=(np.asarray(g["mid_idx"]) + r0).astype(np.int32), j_idx=g["j_idx"],
+                            remap=(np.asarray(g["remap"]) + z0).astype(np.int32)))
+        for g in pk["tc"]:
+            g = dict(g)
+            g["i_indices"] = (np.asarray(g["i_indices"]) + r0).astype(np.int32)
+            g["indices"] = (np.asarray(g["indices"]) + z0).astype(np.int32)
+            tc.append(g)
+    return dict(M=int(M), N=int(N), nnz=int(nnz), g1d=g1d, tc=tc)
+
+
+def _blocked_pack(kind, A, opts, width, parts, workers, nnz_per_part):
+    A = _csr_of(A)
+    if parts is None:
+        parts = max(1, int(round(A.nnz / nnz_per_part)))
+    cuts = row_blocks(A, parts)
+    if len(cuts) == 2:
+        return (spmm_pack if kind == "spmm" else sddmm_pack)(A, opts, width)[0]
+    blocks = [A[int(a):int(b)] for a, b in zip(cuts[:-1], cuts[1:])]
+    packs = parallel_map(_block_job, [(kind, B, opts, width) for B in blocks], workers)
+    offs = A.indptr[cuts[:-1]].astype(np.int64)
+    if kind == "spmm":
+        return stack_spmm_packs(packs, cuts, offs, A.shape[0], A.shape[1])
+    return stack_sddmm_packs(packs, cuts, offs, A.shape[0], A.shape[1], A.nnz)
+
+
+def spmm_pack_blocked(A, opts=None, N=32, parts=None, workers=None, nnz_per_part=1 << 21):
+    """spmm_pack for large A: row blocks of ~nnz_per_part nonzeros searched in
+    parallel processes and stacked into one pack (value ids included)."""
+    return _blocked_pack("spmm", A, opts, N, parts, workers, nnz_per_part)
+
+
+def sddmm_pack_blocked(A, opts=None, Kf=32, parts=None, workers=None, nnz_per_part=1 << 21):
+    return _blocked_pack("sddmm", A, opts, Kf, parts, workers, nnz_per_part)
+
+
 def build_spmm_plan(A, blk_dtype="bfloat16", opts=None, N=32):
     """Search + pack + device plan for C = A.B (any N at run time)."""
     from . import _native

@@ -16,9 +16,6 @@ many B (my_fuse_formats.py:1296-1428); this pipeline is the batched-attention
 workload built from its SDDMM and SpMM.
 """
 import math
-import multiprocessing as mp
-import os
-
 import numpy as np
 import scipy.sparse
 import torch
@@ -35,14 +32,8 @@ def _pack_one(args):
 
 
 def build_packs(kind, mats, opts=None, workers=None):
-    """Search + pack every matrix, in parallel processes (fork)."""
-    workers = workers or min(len(mats), max(1, (os.cpu_count() or 1)), 16)
-    jobs = [(kind, A, opts) for A in mats]
-    if workers <= 1 or len(mats) == 1:
-        return [_pack_one(j) for j in jobs]
-    ctx = mp.get_context("fork")
-    with ctx.Pool(workers) as pool:
-        return pool.map(_pack_one, jobs, chunksize=1)
+    """Search + pack every matrix, in parallel worker processes."""
+    return api.parallel_map(_pack_one, [(kind, A, opts) for A in mats], workers)
 
 
 class BatchedSparseAttention(object):
@@ -56,13 +47,13 @@ class BatchedSparseAttention(object):
         self.d = head_dim
         self.device = torch.device(device)
         self.row_off = np.concatenate([[0], np.cumsum(self.lens)])
+        spmm_packs = build_packs("spmm", self.masks, opts, workers)
+        sddmm_packs = build_packs("sddmm", self.masks, opts, workers)
         big = scipy.sparse.block_diag(self.masks, format="csr")
         big.sort_indices()
         self.nnz = big.nnz
         self.indptr = torch.from_numpy(big.indptr.astype(np.int32)).to(self.device)
         self.indices = big.indices.astype(np.int32)
-        spmm_packs = build_packs("spmm", self.masks, opts, workers)
-        sddmm_packs = build_packs("sddmm", self.masks, opts, workers)
         self.spmm_plan = _native.Plan.spmm(spmm_packs, blk_dtype)
         self.sddmm_plan = _native.Plan.sddmm(sddmm_packs, "float32")
         self.scale = 1.0 / math.sqrt(head_dim)

@@ -16,6 +16,8 @@ This pairing is not part of the reference (it has no backward pass); it is
 the config the GCN workload (forward A.(HW), backward A^T SpMM + SDDMM for
 dA) needs.
 """
+import copy
+
 import numpy as np
 import scipy.sparse
 import torch
@@ -47,7 +49,8 @@ class SparseOperand(object):
     computation FP32; "bfloat16" runs the dense blocks in BF16 with FP32
     accumulation).  opts: api.SearchOptions for the tile search."""
 
-    def __init__(self, A, blk_dtype="float32", opts=None, device="cuda", N=32):
+    def __init__(self, A, blk_dtype="float32", opts=None, device="cuda", N=32, workers=None,
+                 nnz_per_part=1 << 21):
         A = scipy.sparse.csr_matrix(A, dtype=np.float32)
         A.sort_indices()
         self.A = A
@@ -55,14 +58,22 @@ class SparseOperand(object):
         self.nnz = A.nnz
         self.device = torch.device(device)
         ones = scipy.sparse.csr_matrix((np.ones(A.nnz, np.float32), A.indices, A.indptr), shape=A.shape)
-        pack, _, _ = api.spmm_pack(ones, opts, N)
-        self.plan = _native.Plan.spmm(pack, blk_dtype)
         T = _transpose_with_ids(A)
         id_map = T.data.astype(np.int64)
-        tones = scipy.sparse.csr_matrix((np.ones(T.nnz, np.float32), T.indices, T.indptr), shape=T.shape)
-        tpack, _, _ = api.spmm_pack(tones, opts, N)
+        # a symmetric pattern has A^T's CSR structure = A's: the search result
+        # for A serves A^T with its value ids mapped through the transpose
+        self.symmetric = (A.shape[0] == A.shape[1] and np.array_equal(T.indptr, A.indptr)
+                          and np.array_equal(T.indices, A.indices))
+        kw = dict(workers=workers, nnz_per_part=nnz_per_part)
+        pack = api.spmm_pack_blocked(ones, opts, N, **kw)
+        if self.symmetric:
+            tpack = copy.deepcopy(pack)
+        else:
+            tones = scipy.sparse.csr_matrix((np.ones(T.nnz, np.float32), T.indices, T.indptr), shape=T.shape)
+            tpack = api.spmm_pack_blocked(tones, opts, N, **kw)
+        spack = api.sddmm_pack_blocked(ones, opts, 32, **kw)
+        self.plan = _native.Plan.spmm(pack, blk_dtype)
         self.plan_t = _native.Plan.spmm(_remap_ids(tpack, id_map), blk_dtype)
-        spack, _, _ = api.sddmm_pack(ones, opts, 32)
         self.sddmm_plan = _native.Plan.sddmm(spack, "float32")
         self.values = torch.from_numpy(A.data.astype(np.float32)).to(self.device)
         # (tensor, data_ptr, version) of the values now in the SpMM plans; the

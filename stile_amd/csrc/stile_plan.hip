@@ -580,26 +580,11 @@ int add_sddmm_pack(SddmmBuild& bd, const StileSddmmPack* pk, int32_t roff, int32
   if (pk->M < 0 || pk->N < 0 || pk->nnz < 0) return fail(STILE_E_INVALID, "negative size");
   if ((pk->num_1d > 0 && !pk->g1d) || (pk->num_tc > 0 && !pk->tc) || pk->num_1d < 0 || pk->num_tc < 0)
     return fail(STILE_E_INVALID, "group array missing");
+  // Each output entry is computed once: TC blocks first (in group and block
+  // order), then the 1D entries not already computed.  An entry covered twice
+  // (overlapping tiles, padding repeats) would otherwise be written by two
+  // threads with different rounding in no fixed order.
   std::vector<unsigned char> seen(pk->nnz, 0);
-  for (int gi = 0; gi < pk->num_1d; ++gi) {
-    const StileSddmm1DGroup& g = pk->g1d[gi];
-    if (g.len < 0 || (g.len > 0 && (!g.mid_idx || !g.j_idx || !g.remap)))
-      return fail(STILE_E_INVALID, "1D group: bad buffers");
-    int32_t len = g.len;
-    // Trailing padding repeats the last entry (my_fuse_formats.py:3586-3588).
-    while (len > 1 && g.mid_idx[len - 1] == g.mid_idx[len - 2] && g.j_idx[len - 1] == g.j_idx[len - 2] &&
-           g.remap[len - 1] == g.remap[len - 2])
-      --len;
-    for (int32_t e = 0; e < len; ++e) {
-      const int32_t r = g.mid_idx[e], c = g.j_idx[e], o = g.remap[e];
-      if (r < 0 || r >= pk->M || c < 0 || c >= pk->N || o < 0 || o >= pk->nnz)
-        return fail(STILE_E_INVALID, "1D group: index out of range");
-      seen[o] = 1;
-      bd.e_row.push_back(r + roff);
-      bd.e_col.push_back(c + coff);
-      bd.e_out.push_back(o + ooff);
-    }
-  }
   for (int gi = 0; gi < pk->num_tc; ++gi) {
     const StileSddmmTcGroup& g = pk->tc[gi];
     if (g.nnzb < 0) return fail(STILE_E_INVALID, "TC group: negative nnzb");
@@ -621,11 +606,29 @@ int add_sddmm_pack(SddmmBuild& bd, const StileSddmmPack* pk, int32_t roff, int32
         const int32_t pos = g.indices_dense[s], o = g.indices[s];
         if (pos < 0 || pos >= 256 || o < 0 || o >= pk->nnz)
           return fail(STILE_E_INVALID, "TC group: sampled entry out of range");
+        if (seen[o]) continue;
         seen[o] = 1;
         bd.b_pos.push_back(pos);
         bd.b_out.push_back(o + ooff);
       }
       bd.b_ptr.push_back((int32_t)bd.b_pos.size());
+    }
+  }
+  for (int gi = 0; gi < pk->num_1d; ++gi) {
+    const StileSddmm1DGroup& g = pk->g1d[gi];
+    if (g.len < 0 || (g.len > 0 && (!g.mid_idx || !g.j_idx || !g.remap)))
+      return fail(STILE_E_INVALID, "1D group: bad buffers");
+    // trailing padding repeats the last entry (my_fuse_formats.py:3586-3588)
+    // and is dropped with the other duplicates
+    for (int32_t e = 0; e < g.len; ++e) {
+      const int32_t r = g.mid_idx[e], c = g.j_idx[e], o = g.remap[e];
+      if (r < 0 || r >= pk->M || c < 0 || c >= pk->N || o < 0 || o >= pk->nnz)
+        return fail(STILE_E_INVALID, "1D group: index out of range");
+      if (seen[o]) continue;
+      seen[o] = 1;
+      bd.e_row.push_back(r + roff);
+      bd.e_col.push_back(c + coff);
+      bd.e_out.push_back(o + ooff);
     }
   }
   for (int32_t o = 0; o < pk->nnz; ++o)

@@ -136,3 +136,41 @@ def test_set_params_scalars():
     for g in p["groups"]:
         if g["template"] == "TensorCore_template":
             assert g["tile_size"] == 16 and g["funit"] == 32 and g["nb"] == -(-128 // g["group_size"])
+
+
+@pytest.mark.parametrize("kind", ["spmm", "sddmm"])
+def test_row_blocked_packs_stack_to_A(kind):
+    """Row-blocked search: the stacked pack covers every nonzero of A exactly
+    once, at its own (row, col), with value ids = A's CSR positions."""
+    A = datasets.rmat_graph(11, seed=4)
+    A.data = np.random.default_rng(0).standard_normal(A.nnz).astype(np.float32)
+    row_of = np.repeat(np.arange(A.shape[0]), np.diff(A.indptr))
+    cuts = api.row_blocks(A, 3)
+    assert len(cuts) == 4 and all(c % 16 == 0 for c in cuts[1:-1])
+    if kind == "spmm":
+        pack = api.spmm_pack_blocked(A, parts=3, workers=2)
+        r, c, v = expand_spmm_pack(pack)
+        got = scipy.sparse.coo_matrix((v, (r, c)), shape=A.shape).tocsr()
+        assert len(r) == A.nnz
+        assert abs(got - A).max() == 0
+        for g in pack["ell"]:
+            ids = g["a_ids"].reshape(-1, g["nnz_cols"])
+            rr = np.repeat(g["indices_reordered_i"], g["nnz_cols"]).reshape(ids.shape)
+            m = ids >= 0
+            assert np.array_equal(row_of[ids[m]], rr[m])
+            assert np.array_equal(A.indices[ids[m]], g["indices_k"][m])
+    else:
+        pack = api.sddmm_pack_blocked(A, parts=3, workers=2)
+        seen = np.zeros(A.nnz, np.int64)
+        for g in pack["g1d"]:
+            assert np.array_equal(row_of[g["remap"]], g["mid_idx"])
+            assert np.array_equal(A.indices[g["remap"]], g["j_idx"])
+            seen[np.unique(g["remap"])] += 1
+        for g in pack["tc"]:
+            for b in range(g["nnzb"]):
+                s0, s1 = g["indptr"][b], g["indptr"][b + 1]
+                pos, out = g["indices_dense"][s0:s1], g["indices"][s0:s1]
+                assert np.array_equal(row_of[out], g["i_indices"][b][pos // 16])
+                assert np.array_equal(A.indices[out], g["j_indices"][b][pos % 16])
+                np.add.at(seen, out, 1)
+        assert seen.min() >= 1
