@@ -110,7 +110,9 @@ def sddmm_pack(A, opts=None, Kf=32):
 def parallel_map(fn, jobs, workers=None):
     """fn over jobs in worker processes: forked while this process has not
     touched the GPU, spawned otherwise (a forked copy of an initialised HIP
-    runtime is not usable)."""
+    runtime is not usable).  Spawned workers re-import the main module, so a
+    script calling this after GPU work needs the `if __name__ == "__main__"`
+    guard."""
     import multiprocessing as mp
     import os
     workers = min(len(jobs), workers or min(os.cpu_count() or 1, 16))
@@ -242,3 +244,58 @@ def build_sddmm_plan_batched(mats, opts=None, Kf=32):
     from . import _native
     packs = [sddmm_pack(A, opts, Kf)[0] for A in mats]
     return _native.Plan.sddmm(packs, "float32")
+
+
+# ---------------------------------------------------------------------------
+# Pack serialisation (.npz).  A pack is what the search produces; saving it
+# skips the search on the next run (a plan is rebuilt from it in
+# milliseconds).
+# ---------------------------------------------------------------------------
+
+_GROUP_LISTS = ("ell", "tc", "g1d")
+
+
+def save_pack(path, pack):
+    """Write an SpMM or SDDMM pack (the dicts of spmm_pack / sddmm_pack /
+    the blocked variants) to an .npz file: array fields as arrays, scalar
+    fields and group counts as a JSON header."""
+    import json
+    arrays, meta = {}, {"scalars": {}, "groups": {}}
+    for k, v in pack.items():
+        if k in _GROUP_LISTS:
+            meta["groups"][k] = []
+            for i, g in enumerate(v):
+                gm = {}
+                for f, x in g.items():
+                    if x is None:
+                        gm[f] = None
+                    elif isinstance(x, np.ndarray):
+                        arrays[f"{k}{i}.{f}"] = x
+                        gm[f] = "array"
+                    else:
+                        gm[f] = x.item() if isinstance(x, np.generic) else x
+                meta["groups"][k].append(gm)
+        elif k == "order":
+            meta["order"] = [list(o) for o in v]
+        else:
+            meta["scalars"][k] = int(v)
+    arrays["__meta__"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    np.savez(path, **arrays)
+
+
+def load_pack(path):
+    """Read a pack written by save_pack."""
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        meta = json.loads(bytes(z["__meta__"]).decode())
+        pack = dict(meta["scalars"])
+        for k, groups in meta["groups"].items():
+            pack[k] = []
+            for i, gm in enumerate(groups):
+                g = {}
+                for f, x in gm.items():
+                    g[f] = np.array(z[f"{k}{i}.{f}"]) if x == "array" else x
+                pack[k].append(g)
+        if "order" in meta:
+            pack["order"] = [tuple(o) for o in meta["order"]]
+    return pack

@@ -920,3 +920,55 @@ def init_nnz(tile):
     elif tpl == '1D_sddmm':
         b = tile.tile_sizes[0]
         tile.nnz = min((tile.tile_pos[0] + 1) * b, len(op._ent_id)) - tile.tile_pos[0] * b
+
+
+# ---------------------------------------------------------------------------
+# On-disk graphs.
+# ---------------------------------------------------------------------------
+
+def load_snap_graph(op_type, name="arxiv", feat_size=32, pad=True, delimiter=None, seed=0):
+    """Op from a SNAP-style edge list file (gen_formats_v2.py:2629-2705).
+
+    '.txt' files are whitespace separated ints, '.csv' comma separated, any
+    other name is read as floats with `delimiter`; only the first two columns
+    are used.  Duplicate edges are dropped, node ids are relabelled to
+    0..n-1 in sorted order of the ids that occur, and the CSR of ones has
+    shape (ceil(rows/128)*128, ceil(cols/128)*128) for spmm and
+    (ceil(n/128)*128, ceil(n/160)*160) for sddmm when pad, (n, n) otherwise.
+    Dense operands are U[0,1) FP32 from default_rng(seed), as in the
+    reference (B: K x feat for spmm; X: M x feat and Y: K x feat for sddmm).
+    """
+    if ".txt" in name:
+        a = np.loadtxt(name, dtype=np.int64, comments="#", ndmin=2)
+    elif ".csv" in name:
+        a = np.loadtxt(name, dtype=np.int64, delimiter=",", comments="#", ndmin=2)
+    else:
+        a = np.loadtxt(name, dtype=np.float64, delimiter=delimiter, comments="#", ndmin=2)
+    a = a[:, :2].astype(np.int64)
+    a = np.unique(a, axis=0).reshape(-1)
+    _, ids = np.unique(a, return_inverse=True)
+    starts, ends = ids.reshape(-1, 2).T
+    node_num = int(max(starts.max(), ends.max())) + 1
+    A = scipy.sparse.coo_matrix((np.ones(len(starts), np.float32), (starts, ends)),
+                                shape=(node_num, node_num)).tocsr()
+    if pad:
+        rows = A.shape[0]
+        cols = int(A.indices.max()) + 1 if A.nnz else 0
+        if op_type == 'spmm':
+            idx_lens = [math.ceil(rows / 128) * 128, feat_size, math.ceil(cols / 128) * 128]
+        else:
+            idx_lens = [math.ceil(node_num / 128) * 128, feat_size, math.ceil(node_num / 160) * 160]
+    else:
+        idx_lens = [node_num, feat_size, node_num]
+    indptr = np.concatenate([A.indptr, np.full(idx_lens[0] + 1 - len(A.indptr), A.indptr[-1])])
+    A = scipy.sparse.csr_matrix((np.ones(A.nnz, np.float32), A.indices, indptr), shape=(idx_lens[0], idx_lens[2]))
+    A.sum_duplicates()
+    A.data[:] = 1.0
+    A = _as_csr(A)
+    rng = np.random.default_rng(seed=seed)
+    if op_type == 'spmm':
+        Bs = [rng.random((idx_lens[2], feat_size)).astype(np.float32)]
+    else:
+        Bs = [rng.random((idx_lens[0], feat_size)).astype(np.float32),
+              rng.random((idx_lens[2], feat_size)).astype(np.float32)]
+    return Op(op_type, [0, 1], [2], idx_lens, None, [A] + [DenseTensor(B) for B in Bs], [A.tocsc()])

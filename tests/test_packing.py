@@ -174,3 +174,42 @@ def test_row_blocked_packs_stack_to_A(kind):
                 assert np.array_equal(A.indices[out], g["j_indices"][b][pos % 16])
                 np.add.at(seen, out, 1)
         assert seen.min() >= 1
+
+
+@pytest.mark.parametrize("kind", ["spmm", "sddmm"])
+def test_pack_save_load_roundtrip(tmp_path, kind):
+    A = datasets.pruned_layer(seed=2, M=128, K=96, N=8)[0]
+    pack = api.spmm_pack(A)[0] if kind == "spmm" else api.sddmm_pack(A)[0]
+    path = tmp_path / "pack.npz"
+    api.save_pack(path, pack)
+    back = api.load_pack(path)
+    assert set(back) == set(pack)
+    for k, v in pack.items():
+        if isinstance(v, list) and k != "order":
+            assert len(back[k]) == len(v)
+            for g0, g1 in zip(v, back[k]):
+                assert set(g0) == set(g1)
+                for f in g0:
+                    if isinstance(g0[f], np.ndarray):
+                        assert g0[f].dtype == g1[f].dtype and np.array_equal(g0[f], g1[f]), f
+                    else:
+                        assert g0[f] == g1[f], f
+        else:
+            assert back[k] == v
+
+
+def test_load_snap_graph_relabels_dedupes_and_pads(tmp_path):
+    from stile_amd.gen_formats_v2 import load_snap_graph
+    p = tmp_path / "g.txt"
+    p.write_text("# comment\n10 20\n20 30\n10 20\n30 10\n")
+    op = load_snap_graph('spmm', str(p), feat_size=8)
+    A = op.inps[0]
+    # ids 10, 20, 30 -> 0, 1, 2; the duplicate edge dropped; 128-padded
+    assert op.idx_lens == [128, 8, 128]
+    assert A.shape == (128, 128) and A.nnz == 3
+    assert sorted(zip(*A.nonzero())) == [(0, 1), (1, 2), (2, 0)]
+    assert op.inps[1].data.shape == (128, 8)
+    op2 = load_snap_graph('sddmm', str(p), feat_size=8)
+    assert op2.idx_lens == [128, 8, 160]
+    op3 = load_snap_graph('spmm', str(p), feat_size=8, pad=False)
+    assert op3.idx_lens == [3, 8, 3]
