@@ -15,6 +15,12 @@ constexpr int kHubNnz = 4096;
 // the first kSplitNnz nonzeros, the rest go to pieces of kSplitNnz that a
 // fix-up kernel adds with atomicAdd after the window pass.
 constexpr int kSplitNnz = 256;
+// Windows with more than kBigChunks dense chunks are "big": the streaming
+// kernel gives each (big window, 64 columns) a workgroup of kBigWaves waves,
+// each wave a contiguous share of the chunks, and sums the partial results
+// through LDS in wave order.
+constexpr int kBigChunks = 8;
+constexpr int kBigWaves = 8;
 
 // Device view of a hybrid SpMM plan (window-major execution layout).
 //
@@ -32,6 +38,7 @@ constexpr int kSplitNnz = 256;
 struct SpmmPlanDev {
   int32_t M, K, W;
   int32_t W_tc;              // windows [0, W_tc) hold dense chunks; the rest are sparse-only
+  int32_t W_big;             // windows [0, W_big) hold more than kBigChunks chunks
   int32_t nchunks;
   int32_t blk_dtype;
   int32_t npieces;           // split pieces of long sparse rows

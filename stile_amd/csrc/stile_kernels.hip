@@ -307,12 +307,58 @@ hipError_t launch_csr_sddmm_f32(int M, int nnz, int Kf, const int32_t* indptr,
 // ---------------------------------------------------------------------------
 
 // Sum of v * B[col][n0+4j..+3] over nonzeros [r0, r1) of the sparse part.
+#ifndef STILE_SPARSE_DEPTH
+#define STILE_SPARSE_DEPTH 8
+#endif
+
+// DEPTH 8 keeps eight gathers in flight with the next ids prefetched; DEPTH 4
+// (four at a time) is for kernels whose register budget is tight.
+template <int DEPTH>
 __device__ __forceinline__ float4 sparse_row_sum(const SpmmPlanDev& p, const float* __restrict__ B,
                                                  int64_t ldb, int N, int col, bool vec, int r0, int r1,
                                                  bool compensated) {
   float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
   if (!compensated) {
     int q = r0;
+    // Eight gathers in flight, the next eight column ids fetched under them.
+    if (DEPTH >= 8 && q + 8 <= r1) {
+      int k[8];
+      float v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        k[i] = p.res_cols[q + i];
+        v[i] = p.res_vals[q + i];
+      }
+      while (true) {
+        float4 b[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) b[i] = load_b4(B, ldb, k[i], col, N, vec);
+        const bool more = q + 16 <= r1;
+        int kn[8];
+        float vn[8];
+        if (more) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            kn[i] = p.res_cols[q + 8 + i];
+            vn[i] = p.res_vals[q + 8 + i];
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          s.x += v[i] * b[i].x;
+          s.y += v[i] * b[i].y;
+          s.z += v[i] * b[i].z;
+          s.w += v[i] * b[i].w;
+        }
+        q += 8;
+        if (!more) break;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          k[i] = kn[i];
+          v[i] = vn[i];
+        }
+      }
+    }
     for (; q + 4 <= r1; q += 4) {
       const int k0 = p.res_cols[q], k1 = p.res_cols[q + 1], k2 = p.res_cols[q + 2], k3 = p.res_cols[q + 3];
       const float v0 = p.res_vals[q], v1 = p.res_vals[q + 1], v2 = p.res_vals[q + 2], v3 = p.res_vals[q + 3];
@@ -372,6 +418,7 @@ __device__ __forceinline__ SlotInfo load_slots(const SpmmPlanDev& p, int slot0) 
 
 // Add the sparse parts of the 4 slots to the dense accumulators, then store
 // the 4 rows (all loads of the window happen before its first store).
+template <int DEPTH>
 __device__ __forceinline__ void finish_window(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
                                               int N, float* __restrict__ C, int64_t ldc, int col, bool vec,
                                               const SlotInfo& si, const f32x4 (&acc)[4]) {
@@ -382,7 +429,7 @@ __device__ __forceinline__ void finish_window(const SpmmPlanDev& p, const float*
     const int r0 = si.r0[i], rend = si.rend[i];
     if (rend > r0) {
       const int r1 = min(rend, r0 + kSplitNnz);
-      const float4 sp = sparse_row_sum(p, B, ldb, N, col, vec, r0, r1, rend - r0 > kHubNnz);
+      const float4 sp = sparse_row_sum<DEPTH>(p, B, ldb, N, col, vec, r0, r1, rend - r0 > kHubNnz);
       o[i].x += sp.x;
       o[i].y += sp.y;
       o[i].z += sp.z;
@@ -401,7 +448,7 @@ __device__ __forceinline__ void finish_slot(const SpmmPlanDev& p, const float* _
   const int row = p.win_rows[slot];
   const int r0 = p.res_ptr[slot], rend = p.res_ptr[slot + 1];
   const int r1 = min(rend, r0 + kSplitNnz);
-  float4 s = sparse_row_sum(p, B, ldb, N, col, vec, r0, r1, rend - r0 > kHubNnz);
+  float4 s = sparse_row_sum<STILE_SPARSE_DEPTH>(p, B, ldb, N, col, vec, r0, r1, rend - r0 > kHubNnz);
   if (row >= 0) {
     float4 o;
     o.x = dense.x + s.x;
@@ -468,9 +515,9 @@ __device__ __forceinline__ void load_chunk_b(const float* __restrict__ B, int64_
 // latency instead of two.
 template <int MODE>
 __device__ __forceinline__ void window_dense(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
-                                             int N, int col, bool vec, int w, int lane, f32x4 (&acc)[4]) {
+                                             int N, int col, bool vec, int c0, int c1, int lane,
+                                             f32x4 (&acc)[4]) {
   const int g = lane >> 4;
-  const int c0 = p.tc_ptr[w], c1 = p.tc_ptr[w + 1];
   if (c0 >= c1) return;
   auto cols_of = [&](int c, int4& ka, int4& kb) {
     const int4* cp = reinterpret_cast<const int4*>(p.tc_cols + (int64_t)c * 32 + g * 8);
@@ -512,31 +559,70 @@ __device__ __forceinline__ void hyb_spmm_window(const SpmmPlanDev& p, const floa
   f32x4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  window_dense<MODE>(p, B, ldb, N, col, vec, w, lane, acc);
-  finish_window(p, B, ldb, N, C, ldc, col, vec, si, acc);
+  window_dense<MODE>(p, B, ldb, N, col, vec, p.tc_ptr[w], p.tc_ptr[w + 1], lane, acc);
+  finish_window<8>(p, B, ldb, N, C, ldc, col, vec, si, acc);
 }
 
-// Waves [wave_base, ...): first the dense windows (one wave per window and
-// 64 columns), then the sparse-only windows (four waves per window and 64
-// columns).
+// Blocks [0, big_blocks) each run one (big window, 64 columns): wave v takes
+// the v-th contiguous share of the window's chunks, waves 1.. leave their
+// partial sums in LDS and wave 0 adds them in wave order, then adds the
+// sparse parts and stores the 16 rows.
+// The other blocks run kBigWaves waves each over the remaining dense windows
+// [W_big, W_tc), one wave per window and 64 columns.
 template <int MODE>
-__global__ __launch_bounds__(256) void hyb_spmm_kernel(SpmmPlanDev p, const float* __restrict__ B,
-                                                       int64_t ldb, int N, float* __restrict__ C,
-                                                       int64_t ldc, int nC, int wave_base, bool vec) {
-  const int wave = wave_base + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int dense_waves = p.W_tc * nC;
-  if (wave < dense_waves) {
-    const int w = wave / nC;
-    hyb_spmm_window<MODE>(p, B, ldb, N, C, ldc, w, (wave - w * nC) * 64, vec);
+__global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p, const float* __restrict__ B,
+                                                                  int64_t ldb, int N, float* __restrict__ C,
+                                                                  int64_t ldc, int nC, int big_blocks,
+                                                                  bool vec) {
+  __shared__ f32x4 part[kBigWaves - 1][4][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  if ((int)blockIdx.x < big_blocks) {
+    const int w = blockIdx.x / nC;
+    const int col = (blockIdx.x - w * nC) * 64 + 4 * (lane & 15);
+    const int c0 = p.tc_ptr[w], c1 = p.tc_ptr[w + 1];
+    const int share = (c1 - c0 + kBigWaves - 1) / kBigWaves;
+    const int a = min(c0 + wv * share, c1), b = min(a + share, c1);
+    SlotInfo si;
+    if (wv == 0) si = load_slots(p, w * 16 + (lane >> 4) * 4);
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    window_dense<MODE>(p, B, ldb, N, col, vec, a, b, lane, acc);
+    if (wv > 0) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) part[wv - 1][t][lane] = acc[t];
+    }
+    __syncthreads();
+    if (wv == 0) {
+      for (int v = 0; v < kBigWaves - 1; ++v) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] += part[v][t][lane];
+      }
+      finish_window<8>(p, B, ldb, N, C, ldc, col, vec, si, acc);
+    }
     return;
   }
-  const int idx = wave - dense_waves;
-  const int w = p.W_tc + idx / (4 * nC);
-  if (w >= p.W) return;
-  const int rest = idx - (w - p.W_tc) * 4 * nC;
+  const int64_t wave = (int64_t)(blockIdx.x - big_blocks) * kBigWaves + wv;
+  if (wave >= (int64_t)(p.W_tc - p.W_big) * nC) return;
+  const int w = p.W_big + (int)(wave / nC);
+  hyb_spmm_window<MODE>(p, B, ldb, N, C, ldc, w, (int)(wave - (int64_t)(w - p.W_big) * nC) * 64, vec);
+}
+
+// Sparse-only windows [W_tc, W): four waves per window and 64 columns, one
+// row per lane group.  A kernel of its own so that its small register
+// footprint sets its occupancy (gathers in flight per CU), not the dense
+// path's.
+__global__ __launch_bounds__(256) void hyb_spmm_sparse_kernel(SpmmPlanDev p, const float* __restrict__ B,
+                                                              int64_t ldb, int N, float* __restrict__ C,
+                                                              int64_t ldc, int nC, bool vec) {
+  const int lane = threadIdx.x & 63;
+  const int64_t idx = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (idx >= (int64_t)(p.W - p.W_tc) * 4 * nC) return;
+  const int w = p.W_tc + (int)(idx / (4 * nC));
+  const int rest = (int)(idx - (int64_t)(w - p.W_tc) * 4 * nC);
   const int quarter = rest & 3;
   const int n0 = (rest >> 2) * 64;
-  const int lane = threadIdx.x & 63;
   const int col = n0 + 4 * (lane & 15);
   finish_slot(p, B, ldb, N, C, ldc, col, vec, w * 16 + quarter * 4 + (lane >> 4), make_float4(0.f, 0.f, 0.f, 0.f));
 }
@@ -676,7 +762,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
       while (cc >= w_next) {  // window w is complete: add its sparse part, store
         const SlotInfo si_next = load_slots(p, (w + 1) * 16 + g * 4);
         const int w_next2 = p.tc_ptr[w + 2];
-        finish_window(p, B, ldb, N, C, ldc, col, vec, si, acc);
+        finish_window<4>(p, B, ldb, N, C, ldc, col, vec, si, acc);
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
         si = si_next;
@@ -688,7 +774,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
 #pragma unroll
     for (int u = 0; u < kDepth; ++u) cur[u] = nxt[u];
   }
-  finish_window(p, B, ldb, N, C, ldc, col, vec, si, acc);
+  finish_window<4>(p, B, ldb, N, C, ldc, col, vec, si, acc);
 }
 
 // Remaining pieces of long sparse rows: lane group g of the wave takes
@@ -763,7 +849,8 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
   bool lds = p.blk_dtype == 1 && p.K <= kLdsMaxK && p.W_tc > 0;
   if (choice == 1) lds = false;
   if (choice == 2) lds = p.blk_dtype == 1 && p.K <= kLdsMaxK && p.W_tc > 0;
-  int64_t wave_base = 0;
+  // The LDS kernel runs every dense window; otherwise the streaming kernel
+  // does (its first blocks the big windows).  The sparse-only windows follow.
   if (lds) {
     const int g = std::min(groups, p.W_tc);
     const int wpg = (p.W_tc + g - 1) / g;
@@ -772,22 +859,32 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
                        ldc, nC, wpg, vec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    wave_base = (int64_t)p.W_tc * nC;
-  }
-  const int64_t waves = (int64_t)p.W_tc * nC + (int64_t)(p.W - p.W_tc) * nC * 4 - wave_base;
-  if (waves > 0) {
-    dim3 grid((unsigned)((waves + 3) / 4));
+  } else if (p.W_tc > 0) {
+    const int64_t big_blocks = (int64_t)p.W_big * nC;
+    const int64_t blocks = big_blocks + ((int64_t)(p.W_tc - p.W_big) * nC + kBigWaves - 1) / kBigWaves;
+    if (blocks > INT32_MAX) return hipErrorInvalidValue;
+    dim3 grid((unsigned)blocks), block(kBigWaves * 64);
+    const int bb = (int)big_blocks;
     switch (p.blk_dtype) {
       case 1:
-        hipLaunchKernelGGL(hyb_spmm_kernel<1>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, (int)wave_base, vec);
+        hipLaunchKernelGGL(hyb_spmm_kernel<1>, grid, block, 0, s, p, B, ldb, N, C, ldc, nC, bb, vec);
         break;
       case 2:
-        hipLaunchKernelGGL(hyb_spmm_kernel<2>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, (int)wave_base, vec);
+        hipLaunchKernelGGL(hyb_spmm_kernel<2>, grid, block, 0, s, p, B, ldb, N, C, ldc, nC, bb, vec);
         break;
       default:
-        hipLaunchKernelGGL(hyb_spmm_kernel<0>, grid, dim3(256), 0, s, p, B, ldb, N, C, ldc, nC, (int)wave_base, vec);
+        hipLaunchKernelGGL(hyb_spmm_kernel<0>, grid, block, 0, s, p, B, ldb, N, C, ldc, nC, bb, vec);
         break;
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const int64_t sparse_waves = (int64_t)(p.W - p.W_tc) * nC * 4;
+  if (sparse_waves > 0) {
+    const int64_t blocks = (sparse_waves + 3) / 4;
+    if (blocks > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(hyb_spmm_sparse_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, B, ldb, N, C, ldc, nC,
+                       vec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

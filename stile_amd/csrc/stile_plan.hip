@@ -330,13 +330,20 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
     }
     cols.resize(o);
   }
-  // Windows with dense chunks first (W_tc of them), sparse-only windows after;
-  // the kernels map the two kinds onto waves differently.
+  // Windows with more than stile::kBigChunks dense chunks first (most chunks first,
+  // W_big of them), then the other windows with dense chunks (W_tc in all),
+  // then the sparse-only windows; the kernels map the kinds onto waves
+  // differently.
   std::vector<int32_t> order(W);
   for (int32_t w = 0; w < W; ++w) order[w] = w;
+  auto nch = [&](int32_t w) { return (int64_t)(bd.win_cols[w].size() + 31) / 32; };
   std::stable_partition(order.begin(), order.end(), [&](int32_t w) { return !bd.win_cols[w].empty(); });
   int32_t W_tc = 0;
   while (W_tc < W && !bd.win_cols[order[W_tc]].empty()) ++W_tc;
+  auto big_end = std::stable_partition(order.begin(), order.begin() + W_tc,
+                                       [&](int32_t w) { return nch(w) > stile::kBigChunks; });
+  std::stable_sort(order.begin(), big_end, [&](int32_t a, int32_t b) { return nch(a) > nch(b); });
+  const int32_t W_big = (int32_t)(big_end - order.begin());
   {
     std::vector<int32_t> new_of(W);
     for (int32_t nw = 0; nw < W; ++nw) new_of[order[nw]] = nw;
@@ -501,6 +508,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.K = bd.K;
   d.W = W;
   d.W_tc = W_tc;
+  d.W_big = W_big;
   d.nchunks = (int32_t)nchunks;
   d.npieces = (int32_t)piece_row.size();
   d.piece_row = dptr<int32_t>(plan, o_prow);

@@ -98,3 +98,35 @@ def test_spmm_pipeline_batched_uneven(cuda):
     plan = api.build_spmm_plan_batched(mats, blk_dtype="float32")
     C = plan.run_spmm(_t(np.vstack(Bs), cuda)).cpu().numpy()
     _check_fp32(C, scipy.sparse.block_diag(mats, format="csr"), np.vstack(Bs))
+
+
+@pytest.mark.parametrize("blk", ["float32", "bfloat16", "float16"])
+@pytest.mark.parametrize("N", [64, 100, 256])
+def test_spmm_big_windows_split_over_a_workgroup(cuda, blk, N):
+    """Windows with far more than 8 dense chunks (K = 2048, 5 % density, all
+    TC tiles) run as workgroups that split the chunks over 8 waves."""
+    A = scipy.sparse.random(100, 2048, density=0.05, format="csr", random_state=7, dtype=np.float32)
+    B = np.random.default_rng(7).standard_normal((2048, N), dtype=np.float32)
+    print(f"big windows blk={blk} N={N}", flush=True)
+    plan = api.build_spmm_plan(A, blk_dtype=blk, opts=api.SearchOptions(only_TC=True), N=N)
+    info = plan.info()
+    assert info["num_chunks"] >= 8 * info["num_windows"]
+    C = plan.run_spmm(_t(B, cuda)).cpu().numpy()
+    if blk == "float32":
+        _check_fp32(C, A, B)
+    else:
+        assert ops.max_rel_err(C, ops.spmm_fp64(A, B)) <= 1e-2
+    C2 = plan.run_spmm(_t(B, cuda)).cpu().numpy()
+    assert np.array_equal(C, C2), "split windows must sum in a fixed order"
+
+
+def test_plan_from_saved_pack(cuda, tmp_path):
+    A, B = datasets.pruned_layer(seed=3, M=512, K=256, N=64)
+    pack = api.spmm_pack(A)[0]
+    api.save_pack(tmp_path / "p.npz", pack)
+    from stile_amd import _native
+    print("plan from saved pack", flush=True)
+    p0 = _native.Plan.spmm(pack, "float32")
+    p1 = _native.Plan.spmm(api.load_pack(tmp_path / "p.npz"), "float32")
+    Bd = _t(B, cuda)
+    assert np.array_equal(p0.run_spmm(Bd).cpu().numpy(), p1.run_spmm(Bd).cpu().numpy())
