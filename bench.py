@@ -4,7 +4,8 @@ Default workload (one step = one launch of the fused SpMM):
   C = A . B, A the 3072 x 768 block-pruned layer (10 % of the 16x16 blocks
   dense + 1 % unstructured residual, 257,265 nonzeros, seed 0), B 768 x 4096
   FP32 resident in HBM, C 3072 x 4096 FP32.  Tiles from the STile search
-  (greedy + withdraw, cost model); dense blocks run BF16 MFMA with FP32
+  (greedy + withdraw, cost model), the search setting picked by measured
+  latency (api.tune_plan); dense blocks run BF16 MFMA with FP32
   accumulation, the sparse rest FP32.
 
 Other workloads (--workload), each printed as its own JSON line:
@@ -151,6 +152,11 @@ def _spmm_comparators(A, Bd, steps, warmup):
     return ms_csr, ms_vendor
 
 
+def progress(msg):
+    """Progress of the long set-up steps (search, tuning) on stderr."""
+    print(msg, file=sys.stderr, flush=True)
+
+
 def _gf(flops, ms):
     return None if ms is None else round(flops / (ms * 1e-3) / 1e9, 2)
 
@@ -168,7 +174,7 @@ def run_pruned_layer(args, dev):
     A, B = datasets.pruned_layer(seed=0, N=N)
     M, K = A.shape
     t0 = time.perf_counter()
-    plan = api.build_spmm_plan(A, blk_dtype="bfloat16")
+    plan, _, tuning = api.tune_plan("spmm", A, "bfloat16", N, log=progress)
     t_plan = time.perf_counter() - t0
     info = plan.info()
     Bd = torch.from_numpy(B).to(dev)
@@ -197,7 +203,7 @@ def run_pruned_layer(args, dev):
                      "hbm_frac": roofline(spmm_alg_bytes(A.nnz, M, K, 128), ms128)["frac"]},
                plan={"windows": info["num_windows"], "mfma_chunks": info["num_chunks"],
                      "block_nnz": info["block_nnz"], "sparse_nnz": info["sparse_nnz"],
-                     "search_and_plan_s": round(t_plan, 3)})
+                     "search_tune_and_plan_s": round(t_plan, 3), "tuning": tuning})
     if not args.no_cpu:
         out["max_rel_err_vs_fp64"] = ops.max_rel_err(C_h[:, :256], ops.spmm_fp64(A, B[:, :256]))
         out["cpu_baseline"] = cpu_spmm_baseline(A, B, 512, 10.0, "3072x768")
@@ -212,7 +218,7 @@ def run_uniform_spmm(args, dev):
     M, K = A.shape
     N = B.shape[1]
     t0 = time.perf_counter()
-    plan = api.build_spmm_plan(A, blk_dtype="float32", N=N)
+    plan, _, tuning = api.tune_plan("spmm", A, "float32", N, log=progress)
     t_plan = time.perf_counter() - t0
     Bd = torch.from_numpy(B).to(dev)
     C = torch.empty((M, N), device=dev, dtype=torch.float32)
@@ -226,7 +232,7 @@ def run_uniform_spmm(args, dev):
                plain_csr_gflops=_gf(flops, ms_csr), vendor_csr_gflops=_gf(flops, ms_vendor),
                hybrid_over_csr=round(ms_csr / ms, 3),
                hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),
-               plan=dict(plan.info(), search_and_plan_s=round(t_plan, 3)))
+               plan=dict(plan.info(), search_tune_and_plan_s=round(t_plan, 3), tuning=tuning))
     if not args.no_cpu:
         out["max_rel_err_vs_fp64"] = ops.max_rel_err(C.cpu().numpy(), ops.spmm_fp64(A, B))
         out["cpu_baseline"] = cpu_spmm_baseline(A, B, N, 10.0, "4096x4096")
@@ -242,7 +248,7 @@ def run_rmat_sddmm(args, dev):
     M, Nc = A.shape
     Kf = X.shape[1]
     t0 = time.perf_counter()
-    plan = api.build_sddmm_plan(A, Kf=Kf)
+    plan, _, tuning = api.tune_plan("sddmm", A, "float32", Kf, log=progress)
     t_plan = time.perf_counter() - t0
     Xd, Yd = torch.from_numpy(X).to(dev), torch.from_numpy(Y).to(dev)
     out_d = torch.empty(A.nnz, device=dev, dtype=torch.float32)
@@ -256,7 +262,7 @@ def run_rmat_sddmm(args, dev):
                 "M": M, "N": Nc, "K": Kf, "nnz": int(A.nnz), "parallelism": "single GPU"},
                roofline=roofline(sddmm_alg_bytes(A.nnz, M, Nc, Kf), ms),
                plain_csr_gflops=_gf(flops, ms_csr), hybrid_over_csr=round(ms_csr / ms, 3),
-               plan=dict(plan.info(), search_and_plan_s=round(t_plan, 3)))
+               plan=dict(plan.info(), search_tune_and_plan_s=round(t_plan, 3), tuning=tuning))
     if not args.no_cpu:
         got = out_d.cpu().numpy()
         ref = ops.sddmm_fp64(A, X, Y)
@@ -283,7 +289,8 @@ def run_gcn(args, dev):
     M = A.shape[0]
     F = H.shape[1]
     t0 = time.perf_counter()
-    op = SparseOperand(A, blk_dtype="float32", device=dev, N=F)
+    progress(f"gcn: graph scale {args.gcn_scale}, nnz {A.nnz}")
+    op = SparseOperand(A, blk_dtype="float32", device=dev, N=F, tune=True, log=progress)
     t_plan = time.perf_counter() - t0
     Hd = torch.from_numpy(H).to(dev).requires_grad_(True)
     Wd = torch.from_numpy(W).to(dev).requires_grad_(True)
@@ -321,7 +328,7 @@ def run_gcn(args, dev):
                            "vendor_csr_spmm": None if ms_vendor is None else round(ms_vendor, 4),
                            "plain_csr_sddmm": round(ms_csr_sd, 4)},
                plan={"spmm": op.plan.info(), "sddmm": op.sddmm_plan.info(), "symmetric_reuse": op.symmetric,
-                     "search_and_plan_s": round(t_plan, 3)})
+                     "search_tune_and_plan_s": round(t_plan, 3), "tuning": op.tuning})
     if not args.no_cpu:
         rng = np.random.default_rng(0)
         rows = rng.choice(M, 2000, replace=False)
@@ -343,7 +350,7 @@ def run_attention(args, dev):
     masks = datasets.attention_masks(count=64)
     d = 64
     t0 = time.perf_counter()
-    att = BatchedSparseAttention(masks, d, blk_dtype="bfloat16", device=dev)
+    att = BatchedSparseAttention(masks, d, blk_dtype="bfloat16", device=dev, tune=True, log=progress)
     t_plan = time.perf_counter() - t0
     n = int(att.row_off[-1])
     g = torch.Generator(device=dev)
@@ -364,7 +371,7 @@ def run_attention(args, dev):
                roofline=roofline(spmm_alg_bytes(att.nnz, n, n, d), ms_sp),
                kernels_ms={"sddmm": round(ms_sd, 4), "spmm": round(ms_sp, 4)},
                plan={"spmm": att.spmm_plan.info(), "sddmm": att.sddmm_plan.info(),
-                     "search_and_plan_s": round(t_plan, 3)})
+                     "search_tune_and_plan_s": round(t_plan, 3), "tuning": att.tuning})
     if not args.no_cpu:
         att.replay()
         torch.cuda.synchronize()

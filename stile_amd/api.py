@@ -7,6 +7,7 @@ post_process_for_withdraw and measures the fused kernel.  These helpers run
 the same sequence and return a device plan (stile_amd._native.Plan) that can
 be reused for any dense width N.
 """
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -68,17 +69,21 @@ def sddmm_op(A, Kf=32, X=None, Y=None):
 def search_tiles(op, opts=None):
     """Greedy search + withdraw post-processing; returns the final tile list."""
     opts = opts or SearchOptions()
+    prev = my_cost_model.get_profile().name
     if opts.profile:
         my_cost_model.set_profile(opts.profile)
-    if op.op_type == 'spmm':
-        mbs = opts.max_bucket_size or my_search_formats.estimate_max_bucket_size(op)
-    else:
-        mbs = opts.max_bucket_size or 32
-    max_avg = float('inf') if opts.only_ELL else opts.max_avg_cost_diff
-    res = my_search_formats.greedy_search_use_cost_model_lower_bound(
-        op, opts.dsize, {}, mbs, ['A'], ([1], [1], [1]), opts.only_TC, opts.only_ELL, opts.penalty,
-        opts.TC_k_notsorted, reorder_k_by_nnz=opts.reorder_k_by_nnz, max_level_num=opts.max_level_num,
-        max_avg_cost_diff=max_avg, no_withdraw=opts.no_withdraw)
+    try:
+        if op.op_type == 'spmm':
+            mbs = opts.max_bucket_size or my_search_formats.estimate_max_bucket_size(op)
+        else:
+            mbs = opts.max_bucket_size or 32
+        max_avg = float('inf') if opts.only_ELL else opts.max_avg_cost_diff
+        res = my_search_formats.greedy_search_use_cost_model_lower_bound(
+            op, opts.dsize, {}, mbs, ['A'], ([1], [1], [1]), opts.only_TC, opts.only_ELL, opts.penalty,
+            opts.TC_k_notsorted, reorder_k_by_nnz=opts.reorder_k_by_nnz, max_level_num=opts.max_level_num,
+            max_avg_cost_diff=max_avg, no_withdraw=opts.no_withdraw)
+    finally:
+        my_cost_model.set_profile(prev)
     tiles, row_dict, row_costs, rev, _levels = res
     tiles = my_branch_and_bound.post_process_for_withdraw(op, tiles, row_dict, row_costs, rev)
     return tiles
@@ -220,10 +225,19 @@ def sddmm_pack_blocked(A, opts=None, Kf=32, parts=None, workers=None, nnz_per_pa
     return _blocked_pack("sddmm", A, opts, Kf, parts, workers, nnz_per_part)
 
 
+def options_for(blk_dtype, opts=None):
+    """Search options for plans of the given block type: the cost model's
+    element size (dsize) follows the blocks (32 for FP32, 16 for BF16/FP16)
+    unless opts sets it."""
+    if opts is not None:
+        return opts
+    return SearchOptions(dsize=32 if blk_dtype in ("float32", "f32", 0) else 16)
+
+
 def build_spmm_plan(A, blk_dtype="bfloat16", opts=None, N=32):
     """Search + pack + device plan for C = A.B (any N at run time)."""
     from . import _native
-    pack, _, _ = spmm_pack(A, opts, N)
+    pack, _, _ = spmm_pack(A, options_for(blk_dtype, opts), N)
     return _native.Plan.spmm(pack, blk_dtype)
 
 
@@ -236,7 +250,7 @@ def build_sddmm_plan(A, opts=None, Kf=32):
 def build_spmm_plan_batched(mats, blk_dtype="bfloat16", opts=None, N=32):
     """One plan over many (uneven) matrices: their block-diagonal composition."""
     from . import _native
-    packs = [spmm_pack(A, opts, N)[0] for A in mats]
+    packs = [spmm_pack(A, options_for(blk_dtype, opts), N)[0] for A in mats]
     return _native.Plan.spmm(packs, blk_dtype)
 
 
@@ -299,3 +313,104 @@ def load_pack(path):
         if "order" in meta:
             pack["order"] = [tuple(o) for o in meta["order"]]
     return pack
+
+
+# ---------------------------------------------------------------------------
+# Measured-latency tuning.  The cost model ranks tiles in isolation; the
+# fused kernels' real cost also depends on B-row reuse in the caches and on
+# how dense and sparse parts share windows.  The tuner therefore searches the
+# matrix under a few settings of the search, builds each plan and keeps the
+# one that runs fastest at the given width (the reference likewise times its
+# candidate formats, my_fuse_formats.measure_latency).
+# ---------------------------------------------------------------------------
+
+def tuning_candidates(blk_dtype, kind="spmm"):
+    """Search settings the tuner tries: the measured MI355X cost model (if
+    present), the reference's A100 model, and the two single-kind extremes."""
+    dsize = 32 if blk_dtype in ("float32", "f32", 0) else 16
+    cands = {}
+    if my_cost_model.get_profile().name == "mi355x" or os.path.exists(my_cost_model.PROFILE_PATH):
+        cands["mi355x"] = SearchOptions(dsize=dsize, profile="mi355x")
+    cands["a100"] = SearchOptions(dsize=dsize, profile="a100")
+    cands["only_tc"] = SearchOptions(dsize=dsize, only_TC=True)
+    cands["only_sparse"] = SearchOptions(dsize=dsize, only_ELL=True)
+    return cands
+
+
+def _pack_any(kind, mats, opts, width, workers):
+    if isinstance(mats, (list, tuple)):
+        job = [(kind, A, opts, width) for A in mats]
+        return parallel_map(_block_job, job, workers)
+    if kind == "spmm":
+        return spmm_pack_blocked(mats, opts, width, workers=workers)
+    return sddmm_pack_blocked(mats, opts, width, workers=workers)
+
+
+def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps=20, workers=None, log=None):
+    """Build the plan of `mats` (one matrix, or a list for a batched plan)
+    under each candidate search setting and return (fastest plan, its pack,
+    report).  kind: 'spmm' (width = N) or 'sddmm' (width = K_feat, FP32
+    blocks).  report: {"choice": name, "ms": {name: ms per run},
+    "search_s": {name: seconds}}."""
+    import time as _time
+    import torch
+    from . import _native
+    if kind == "sddmm":
+        blk_dtype = "float32"
+    cands = candidates or tuning_candidates(blk_dtype, kind)
+    packs, search_s = {}, {}
+    for name, opts in cands.items():          # every search before the GPU is used here
+        t0 = _time.perf_counter()
+        packs[name] = _pack_any(kind, mats, opts, width, workers)
+        search_s[name] = round(_time.perf_counter() - t0, 3)
+        if log:
+            log(f"tune {kind}: searched {name} in {search_s[name]} s")
+    if isinstance(mats, (list, tuple)):
+        M = sum(A.shape[0] for A in mats)
+        K = sum(A.shape[1] for A in mats)
+    else:
+        M, K = mats.shape
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    if kind == "spmm":
+        Bm = torch.randn(K, width, device=dev, generator=g)
+        out = torch.empty(M, width, device=dev)
+    else:
+        X = torch.randn(M, width, device=dev, generator=g)
+        Y = torch.randn(K, width, device=dev, generator=g)
+    best, best_ms, times = None, float("inf"), {}
+    for name, pack in packs.items():
+        plan = _native.Plan.spmm(pack, blk_dtype) if kind == "spmm" else _native.Plan.sddmm(pack, "float32")
+        if kind == "spmm":
+            run = lambda: plan.run_spmm(Bm, out)
+        else:
+            o = torch.empty(pack_nnz(pack), device=dev)
+            run = lambda: plan.run_sddmm(X, Y, o)
+        for _ in range(3):
+            run()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            run()
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / reps
+        times[name] = round(ms, 5)
+        if log:
+            log(f"tune {kind} {name}: {ms * 1e3:.2f} us (search {search_s[name]} s)")
+        if ms < best_ms:
+            if best is not None:
+                best.close()
+            best, best_ms, choice = plan, ms, name
+        else:
+            plan.close()
+    return best, packs[choice], {"choice": choice, "ms": times, "search_s": search_s}
+
+
+def pack_nnz(pack):
+    """Number of outputs of an SDDMM pack (or list of packs)."""
+    if isinstance(pack, (list, tuple)):
+        return sum(int(p["nnz"]) for p in pack)
+    return int(pack["nnz"])

@@ -39,7 +39,8 @@ def build_packs(kind, mats, opts=None, workers=None):
 class BatchedSparseAttention(object):
     """Sparse attention over a list of CSR masks (one per instance)."""
 
-    def __init__(self, masks, head_dim, blk_dtype="bfloat16", opts=None, device="cuda", workers=None):
+    def __init__(self, masks, head_dim, blk_dtype="bfloat16", opts=None, device="cuda", workers=None,
+                 tune=False, log=None):
         self.masks = [scipy.sparse.csr_matrix(m) for m in masks]
         for m in self.masks:
             m.sort_indices()
@@ -47,15 +48,24 @@ class BatchedSparseAttention(object):
         self.d = head_dim
         self.device = torch.device(device)
         self.row_off = np.concatenate([[0], np.cumsum(self.lens)])
-        spmm_packs = build_packs("spmm", self.masks, opts, workers)
-        sddmm_packs = build_packs("sddmm", self.masks, opts, workers)
+        self.tuning = None
+        if tune:
+            # one search setting for all instances, chosen by measured latency
+            ones = [scipy.sparse.csr_matrix((np.ones(m.nnz, np.float32), m.indices, m.indptr), shape=m.shape)
+                    for m in self.masks]
+            self.spmm_plan, _, rep = api.tune_plan("spmm", ones, blk_dtype, head_dim, workers=workers, log=log)
+            self.sddmm_plan, _, srep = api.tune_plan("sddmm", ones, "float32", head_dim, workers=workers, log=log)
+            self.tuning = {"spmm": rep, "sddmm": srep}
+        else:
+            spmm_packs = build_packs("spmm", self.masks, api.options_for(blk_dtype, opts), workers)
+            sddmm_packs = build_packs("sddmm", self.masks, api.options_for("float32", opts), workers)
+            self.spmm_plan = _native.Plan.spmm(spmm_packs, blk_dtype)
+            self.sddmm_plan = _native.Plan.sddmm(sddmm_packs, "float32")
         big = scipy.sparse.block_diag(self.masks, format="csr")
         big.sort_indices()
         self.nnz = big.nnz
         self.indptr = torch.from_numpy(big.indptr.astype(np.int32)).to(self.device)
         self.indices = big.indices.astype(np.int32)
-        self.spmm_plan = _native.Plan.spmm(spmm_packs, blk_dtype)
-        self.sddmm_plan = _native.Plan.sddmm(sddmm_packs, "float32")
         self.scale = 1.0 / math.sqrt(head_dim)
         self.scores = torch.empty(self.nnz, device=self.device, dtype=torch.float32)
         self.probs = torch.empty(self.nnz, device=self.device, dtype=torch.float32)

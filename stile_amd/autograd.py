@@ -50,7 +50,7 @@ class SparseOperand(object):
     accumulation).  opts: api.SearchOptions for the tile search."""
 
     def __init__(self, A, blk_dtype="float32", opts=None, device="cuda", N=32, workers=None,
-                 nnz_per_part=1 << 21):
+                 nnz_per_part=1 << 21, tune=False, log=None):
         A = scipy.sparse.csr_matrix(A, dtype=np.float32)
         A.sort_indices()
         self.A = A
@@ -65,16 +65,31 @@ class SparseOperand(object):
         self.symmetric = (A.shape[0] == A.shape[1] and np.array_equal(T.indptr, A.indptr)
                           and np.array_equal(T.indices, A.indices))
         kw = dict(workers=workers, nnz_per_part=nnz_per_part)
-        pack = api.spmm_pack_blocked(ones, opts, N, **kw)
-        if self.symmetric:
-            tpack = copy.deepcopy(pack)
+        self.tuning = None
+        if tune:
+            # plans chosen by measured latency at width N (api.tune_plan)
+            self.plan, pack, rep = api.tune_plan("spmm", ones, blk_dtype, N, workers=workers, log=log)
+            srep = None
+            if self.symmetric:
+                tpack = copy.deepcopy(pack)
+            else:
+                tones = scipy.sparse.csr_matrix((np.ones(T.nnz, np.float32), T.indices, T.indptr), shape=T.shape)
+                plan_t, tpack, _ = api.tune_plan("spmm", tones, blk_dtype, N, workers=workers, log=log)
+                plan_t.close()
+            self.sddmm_plan, _, srep = api.tune_plan("sddmm", ones, "float32", N, workers=workers, log=log)
+            self.tuning = {"spmm": rep, "sddmm": srep}
         else:
-            tones = scipy.sparse.csr_matrix((np.ones(T.nnz, np.float32), T.indices, T.indptr), shape=T.shape)
-            tpack = api.spmm_pack_blocked(tones, opts, N, **kw)
-        spack = api.sddmm_pack_blocked(ones, opts, 32, **kw)
-        self.plan = _native.Plan.spmm(pack, blk_dtype)
+            opts = api.options_for(blk_dtype, opts)
+            pack = api.spmm_pack_blocked(ones, opts, N, **kw)
+            if self.symmetric:
+                tpack = copy.deepcopy(pack)
+            else:
+                tones = scipy.sparse.csr_matrix((np.ones(T.nnz, np.float32), T.indices, T.indptr), shape=T.shape)
+                tpack = api.spmm_pack_blocked(tones, opts, N, **kw)
+            spack = api.sddmm_pack_blocked(ones, opts, 32, **kw)
+            self.plan = _native.Plan.spmm(pack, blk_dtype)
+            self.sddmm_plan = _native.Plan.sddmm(spack, "float32")
         self.plan_t = _native.Plan.spmm(_remap_ids(tpack, id_map), blk_dtype)
-        self.sddmm_plan = _native.Plan.sddmm(spack, "float32")
         self.values = torch.from_numpy(A.data.astype(np.float32)).to(self.device)
         # (tensor, data_ptr, version) of the values now in the SpMM plans; the
         # tensor is held so its memory cannot be reused by another tensor

@@ -53,6 +53,8 @@ struct SpmmPlanDev {
   const int32_t* res_ptr;    // [W*16+1]
   const int32_t* res_cols;
   const float* res_vals;
+  int32_t n_mix;             // slots of dense windows with a sparse part
+  const int32_t* mix_slot;   // [n_mix]
 };
 
 // Device view of a hybrid SDDMM plan.

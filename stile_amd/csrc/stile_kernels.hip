@@ -301,9 +301,12 @@ hipError_t launch_csr_sddmm_f32(int M, int nnz, int Kf, const int32_t* indptr,
 // FP32 nonzeros of its rows (compensated when a row has more than kHubNnz);
 // the rest of a long row is added by hyb_spmm_fixup_kernel.
 //
-// Windows [0, W_tc) (with dense chunks) take one wave per 64 columns and a
-// lane group runs 4 rows' sparse parts; sparse-only windows take four waves
-// per 64 columns and a lane group runs one row.
+// Kernels of one run, in stream order: the dense windows [0, W_tc) store
+// their dense parts (LDS kernel, or the streaming kernel with one wave per
+// window and 64 columns, a workgroup for windows of more than kBigChunks
+// chunks); the sparse kernel then adds the sparse parts of those windows'
+// rows and writes the sparse-only windows (one row per lane group); the
+// fix-up kernel adds the pieces of long rows.
 // ---------------------------------------------------------------------------
 
 // Sum of v * B[col][n0+4j..+3] over nonzeros [r0, r1) of the sparse part.
@@ -393,52 +396,41 @@ __device__ __forceinline__ float4 sparse_row_sum(const SpmmPlanDev& p, const flo
   return s;
 }
 
-// Output rows and sparse ranges of the 4 slots a lane group finishes.  Loaded
-// before any of the window's C stores: vector-memory counters complete in
-// issue order on CDNA, so a load issued after a store would wait for it.
-struct SlotInfo {
+// Output rows of the 4 slots a lane group stores.  Loaded before any of the
+// window's C stores: vector-memory counters complete in issue order on
+// CDNA, so a load issued after a store would wait for it.
+struct SlotRows {
   int row[4];
-  int r0[4];
-  int rend[4];
 };
 
-__device__ __forceinline__ SlotInfo load_slots(const SpmmPlanDev& p, int slot0) {
-  SlotInfo si;
+__device__ __forceinline__ SlotRows load_rows(const SpmmPlanDev& p, int slot0) {
+  SlotRows sr;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    si.row[i] = p.win_rows[slot0 + i];
-    si.r0[i] = p.res_ptr[slot0 + i];
-  }
-  si.rend[0] = si.r0[1];
-  si.rend[1] = si.r0[2];
-  si.rend[2] = si.r0[3];
-  si.rend[3] = p.res_ptr[slot0 + 4];
-  return si;
+  for (int i = 0; i < 4; ++i) sr.row[i] = p.win_rows[slot0 + i];
+  return sr;
 }
 
-// Add the sparse parts of the 4 slots to the dense accumulators, then store
-// the 4 rows (all loads of the window happen before its first store).
-template <int DEPTH>
-__device__ __forceinline__ void finish_window(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
-                                              int N, float* __restrict__ C, int64_t ldc, int col, bool vec,
-                                              const SlotInfo& si, const f32x4 (&acc)[4]) {
-  float4 o[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    o[i] = make_float4(acc[0][i], acc[1][i], acc[2][i], acc[3][i]);
-    const int r0 = si.r0[i], rend = si.rend[i];
-    if (rend > r0) {
-      const int r1 = min(rend, r0 + kSplitNnz);
-      const float4 sp = sparse_row_sum<DEPTH>(p, B, ldb, N, col, vec, r0, r1, rend - r0 > kHubNnz);
-      o[i].x += sp.x;
-      o[i].y += sp.y;
-      o[i].z += sp.z;
-      o[i].w += sp.w;
-    }
-  }
+// Store the dense accumulators of the lane group's 4 rows (acc[t][i] is row
+// 4g+i, column col+t).  The sparse parts of these rows, if any, are added
+// afterwards by hyb_spmm_sparse_kernel.
+__device__ __forceinline__ void store_window(float* __restrict__ C, int64_t ldc, int N, int col, bool vec,
+                                             const SlotRows& sr, const f32x4 (&acc)[4]) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-    if (si.row[i] >= 0) store_c4(C, ldc, si.row[i], col, N, vec, o[i]);
+    if (sr.row[i] >= 0)
+      store_c4(C, ldc, sr.row[i], col, N, vec, make_float4(acc[0][i], acc[1][i], acc[2][i], acc[3][i]));
+}
+
+// Slot of a dense window with a sparse part: C[row] += first kSplitNnz
+// nonzeros of the part (the dense kernel stored C[row] before).
+__device__ __forceinline__ void add_slot(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb, int N,
+                                         float* __restrict__ C, int64_t ldc, int col, bool vec, int slot) {
+  const int row = p.win_rows[slot];
+  const int r0 = p.res_ptr[slot], rend = p.res_ptr[slot + 1];
+  const int r1 = min(rend, r0 + kSplitNnz);
+  const float4 s = sparse_row_sum<STILE_SPARSE_DEPTH>(p, B, ldb, N, col, vec, r0, r1, rend - r0 > kHubNnz);
+  const float4 c = load_b4(C, ldc, row, col, N, vec);
+  store_c4(C, ldc, row, col, N, vec, make_float4(c.x + s.x, c.y + s.y, c.z + s.z, c.w + s.w));
 }
 
 // One slot (sparse-only windows, one row per lane group).
@@ -555,12 +547,12 @@ __device__ __forceinline__ void hyb_spmm_window(const SpmmPlanDev& p, const floa
   const int j = lane & 15;
   const int col = n0 + 4 * j;
 
-  const SlotInfo si = load_slots(p, w * 16 + g * 4);
+  const SlotRows sr = load_rows(p, w * 16 + g * 4);
   f32x4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
   window_dense<MODE>(p, B, ldb, N, col, vec, p.tc_ptr[w], p.tc_ptr[w + 1], lane, acc);
-  finish_window<8>(p, B, ldb, N, C, ldc, col, vec, si, acc);
+  store_window(C, ldc, N, col, vec, sr, acc);
 }
 
 // Blocks [0, big_blocks) each run one (big window, 64 columns): wave v takes
@@ -583,8 +575,8 @@ __global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p,
     const int c0 = p.tc_ptr[w], c1 = p.tc_ptr[w + 1];
     const int share = (c1 - c0 + kBigWaves - 1) / kBigWaves;
     const int a = min(c0 + wv * share, c1), b = min(a + share, c1);
-    SlotInfo si;
-    if (wv == 0) si = load_slots(p, w * 16 + (lane >> 4) * 4);
+    SlotRows sr;
+    if (wv == 0) sr = load_rows(p, w * 16 + (lane >> 4) * 4);
     f32x4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -599,7 +591,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p,
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[t] += part[v][t][lane];
       }
-      finish_window<8>(p, B, ldb, N, C, ldc, col, vec, si, acc);
+      store_window(C, ldc, N, col, vec, sr, acc);
     }
     return;
   }
@@ -609,15 +601,25 @@ __global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p,
   hyb_spmm_window<MODE>(p, B, ldb, N, C, ldc, w, (int)(wave - (int64_t)(w - p.W_big) * nC) * 64, vec);
 }
 
-// Sparse-only windows [W_tc, W): four waves per window and 64 columns, one
-// row per lane group.  A kernel of its own so that its small register
-// footprint sets its occupancy (gathers in flight per CU), not the dense
-// path's.
+// Sparse parts, one row per lane group: first the n_mix slots of dense
+// windows (four per wave and 64 columns, added to the stored dense rows),
+// then the sparse-only windows [W_tc, W) (four waves per window and 64
+// columns).  A kernel of its own so that its small register footprint sets
+// its occupancy (gathers in flight per CU), not the dense path's.
 __global__ __launch_bounds__(256) void hyb_spmm_sparse_kernel(SpmmPlanDev p, const float* __restrict__ B,
                                                               int64_t ldb, int N, float* __restrict__ C,
                                                               int64_t ldc, int nC, bool vec) {
   const int lane = threadIdx.x & 63;
-  const int64_t idx = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t mix_waves = (int64_t)((p.n_mix + 3) / 4) * nC;
+  if (wave < mix_waves) {
+    const int q = (int)(wave / nC);
+    const int col = (int)(wave - (int64_t)q * nC) * 64 + 4 * (lane & 15);
+    const int m = q * 4 + (lane >> 4);
+    if (m < p.n_mix) add_slot(p, B, ldb, N, C, ldc, col, vec, p.mix_slot[m]);
+    return;
+  }
+  const int64_t idx = wave - mix_waves;
   if (idx >= (int64_t)(p.W - p.W_tc) * 4 * nC) return;
   const int w = p.W_tc + (int)(idx / (4 * nC));
   const int rest = (int)(idx - (int64_t)(w - p.W_tc) * 4 * nC);
@@ -745,7 +747,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
   int w = w_beg;
   int w_next = p.tc_ptr[w + 1];
-  SlotInfo si = load_slots(p, w * 16 + g * 4);
+  SlotRows sr = load_rows(p, w * 16 + g * 4);
   ChunkRegs cur[kDepth];
 #pragma unroll
   for (int u = 0; u < kDepth; ++u)
@@ -759,13 +761,13 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
     for (int u = 0; u < kDepth; ++u) {
       const int cc = c + u;
       if (cc >= c_end) break;
-      while (cc >= w_next) {  // window w is complete: add its sparse part, store
-        const SlotInfo si_next = load_slots(p, (w + 1) * 16 + g * 4);
+      while (cc >= w_next) {  // window w is complete: store it
+        const SlotRows sr_next = load_rows(p, (w + 1) * 16 + g * 4);
         const int w_next2 = p.tc_ptr[w + 2];
-        finish_window<4>(p, B, ldb, N, C, ldc, col, vec, si, acc);
+        store_window(C, ldc, N, col, vec, sr, acc);
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        si = si_next;
+        sr = sr_next;
         ++w;
         w_next = w_next2;
       }
@@ -774,7 +776,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
 #pragma unroll
     for (int u = 0; u < kDepth; ++u) cur[u] = nxt[u];
   }
-  finish_window<4>(p, B, ldb, N, C, ldc, col, vec, si, acc);
+  store_window(C, ldc, N, col, vec, sr, acc);
 }
 
 // Remaining pieces of long sparse rows: lane group g of the wave takes
@@ -879,7 +881,7 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  const int64_t sparse_waves = (int64_t)(p.W - p.W_tc) * nC * 4;
+  const int64_t sparse_waves = (int64_t)((p.n_mix + 3) / 4) * nC + (int64_t)(p.W - p.W_tc) * nC * 4;
   if (sparse_waves > 0) {
     const int64_t blocks = (sparse_waves + 3) / 4;
     if (blocks > INT32_MAX) return hipErrorInvalidValue;

@@ -392,6 +392,11 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
       if (bd.has_ids) res_ids[q] = bd.s_id[src[q]];
     }
   }
+  // Slots of dense windows that also have a sparse part: the sparse kernel
+  // adds those parts to the rows the dense kernels stored.
+  std::vector<int32_t> mix_slot;
+  for (int64_t sl = 0; sl < (int64_t)W_tc * 16; ++sl)
+    if (cnt[sl + 1] > cnt[sl]) mix_slot.push_back((int32_t)sl);
   // Long rows: the window pass takes the first kSplitNnz nonzeros of a slot,
   // the rest is cut into pieces of kSplitNnz for the fix-up kernel.
   int32_t hub_rows = 0;
@@ -477,6 +482,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const size_t o_pend = lay.add(piece_end.size() * 4);
   const size_t o_tcids = lay.add(tc_ids.size() * 4);
   const size_t o_resids = lay.add(res_ids.size() * 4);
+  const size_t o_mix = lay.add(mix_slot.size() * 4);
   HostBlob blob;
   blob.data.assign(lay.bytes, 0);
   blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
@@ -492,6 +498,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   blob.put(o_pend, piece_end.data(), piece_end.size() * 4);
   blob.put(o_tcids, tc_ids.data(), tc_ids.size() * 4);
   blob.put(o_resids, res_ids.data(), res_ids.size() * 4);
+  blob.put(o_mix, mix_slot.data(), mix_slot.size() * 4);
 
   StilePlan* plan = new (std::nothrow) StilePlan();
   if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
@@ -509,6 +516,8 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.W = W;
   d.W_tc = W_tc;
   d.W_big = W_big;
+  d.n_mix = (int32_t)mix_slot.size();
+  d.mix_slot = dptr<int32_t>(plan, o_mix);
   d.nchunks = (int32_t)nchunks;
   d.npieces = (int32_t)piece_row.size();
   d.piece_row = dptr<int32_t>(plan, o_prow);

@@ -109,7 +109,7 @@ def _tc_tile_sizes(op, max_bucket_size=None):
 def _tc_cost(op, tile_sizes, dsize, is_atomic, penalty):
     tpl = 'TensorCore_template' if op.op_type == 'spmm' else 'TC_sddmm'
     return my_cost_model.get_benchmark_cost_(tpl, tile_sizes, dsize, is_atomic if op.op_type == 'spmm' else False,
-                                             penalty)
+                                             penalty, b_rows=op.idx_lens[2])
 
 
 def _make_tc_tile(op, tile_sizes, pos, is_atomic, pred_cost):

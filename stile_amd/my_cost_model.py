@@ -18,7 +18,8 @@ in a HardwareProfile:
            (my_cost_model.py:672-761, 786-800, 1238-1245; PEAK ratios
            19.49/1.555, 78/1.555, 312/1.555; SM_num 108)
   'mi355x' constants measured with this library's kernels on an MI355X by
-           stile_amd.autotune (tables written to mi355x_profile.json)
+           stile_amd.autotune (tables written to mi355x_profile.json); the
+           default once that file exists
 The MLP cost model of the reference (env other_cost_model=MLP, :925-1093)
 is not built.
 """
@@ -48,6 +49,13 @@ class HardwareProfile:
     ell_cost: dict
     ell_cost_atomic: dict
     tc_sddmm_cost: dict
+    tc_cost_f32: dict = None   # TC table for dsize 32 (FP32 blocks); tc_cost serves both if None
+    tc_cost_lds: dict = None   # TC table for BF16 blocks when B has <= LDS_MAX_K rows (B-stationary kernel)
+
+
+# B-stationary SpMM kernel: BF16 blocks and at most this many rows of B
+# (hyb_spmm_lds_bf16_kernel stages B[:, 64 columns] in LDS).
+LDS_MAX_K = 1024
 
 
 _A100 = HardwareProfile(
@@ -90,8 +98,9 @@ _A100 = HardwareProfile(
 )
 
 _PROFILES = {"a100": _A100}
-_ACTIVE = ["a100"]
 PROFILE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mi355x_profile.json")
+# The measured MI355X profile is the default once it exists.
+_ACTIVE = ["mi355x" if os.path.exists(PROFILE_PATH) else "a100"]
 
 
 def _key_tuple(k):
@@ -104,6 +113,8 @@ def load_profile_file(path=PROFILE_PATH):
         d = json.load(f)
     conv = lambda tab: {_key_tuple(k): float(v) for k, v in tab.items()}
     prof = HardwareProfile(
+        tc_cost_f32=conv(d["tc_cost_f32"]) if d.get("tc_cost_f32") else None,
+        tc_cost_lds=conv(d["tc_cost_lds"]) if d.get("tc_cost_lds") else None,
         name=d["name"], sm_num=int(d["sm_num"]), threads_per_sm=int(d["threads_per_sm"]),
         peak_fp32=float(d["peak_fp32"]), peak_fp16=float(d["peak_fp16"]), peak_tc=float(d["peak_tc"]),
         ell_coeff={int(k): tuple(v) for k, v in d["ell_coeff"].items()},
@@ -116,9 +127,10 @@ def load_profile_file(path=PROFILE_PATH):
     return prof
 
 
-def save_profile_file(prof, path=PROFILE_PATH):
+def save_profile_file(prof, path=PROFILE_PATH, meta=None):
     enc = lambda tab: {str(tuple(k)): v for k, v in tab.items()}
-    d = dict(name=prof.name, sm_num=prof.sm_num, threads_per_sm=prof.threads_per_sm,
+    d = dict(meta=meta or {}, tc_cost_f32=enc(prof.tc_cost_f32) if prof.tc_cost_f32 else None,
+             tc_cost_lds=enc(prof.tc_cost_lds) if prof.tc_cost_lds else None, name=prof.name, sm_num=prof.sm_num, threads_per_sm=prof.threads_per_sm,
              peak_fp32=prof.peak_fp32, peak_fp16=prof.peak_fp16, peak_tc=prof.peak_tc,
              ell_coeff={str(k): list(v) for k, v in prof.ell_coeff.items()},
              ell_atomic_coeff={str(k): list(v) for k, v in prof.ell_atomic_coeff.items()},
@@ -176,21 +188,36 @@ def _tile_unique_cols(op, I, pos_is):
     return cnt, uniq, has0
 
 
-def memory_cost_tb_given_tile_sizes_and_poses(tile_sizes, tile_pos_is, template_str, dsize, op):
-    """Memory cost in 32-byte units of ELL tiles at positions tile_pos_is:
-    [non-atomic, atomic] (reference :517-582)."""
+def _ell_mem_units(I, J, K, cnt, uniq, has0, dsize):
+    """[non-atomic, atomic] memory cost in 32-byte units of ELL tiles with
+    cnt nonzeros on uniq distinct columns (has0: column 0 among them; a tile
+    with padding reads column 0 for its padded slots)."""
     cache_line_size = 32 * 8 / dsize
-    if template_str != "sparse_template_ell":
-        raise AssertionError("We only support ELL tiles now.")
-    I, J, K = tile_sizes[0][1], tile_sizes[1][1], tile_sizes[2][1]
     memory_cost_A = math.ceil(I * K / cache_line_size)
-    pos_is = np.asarray(tile_pos_is, dtype=np.int64)
-    cnt, uniq, has0 = _tile_unique_cols(op, I, pos_is)
-    pads = cnt < I * K
-    k_is = uniq + (pads & ~has0)
+    pads = np.asarray(cnt) < I * K
+    k_is = np.asarray(uniq) + (pads & ~np.asarray(has0, bool))
     memory_cost_B = math.ceil(J / cache_line_size) * np.asarray(k_is)
     store_cost = math.ceil(J * dsize / (32 * 8)) * I
     return [memory_cost_A + store_cost + memory_cost_B, memory_cost_A + store_cost * 2 + memory_cost_B]
+
+
+def ell_tile_x(I, J, K, cnt, uniq, has0, dsize, peak):
+    """The latency feature of non-atomic ELL tiles: FLOPs over the roofline
+    throughput min(FLOPs / bytes, peak) (cost_tb_latency_given_ELL_tiles)."""
+    compute_cost = I * J * K * 2
+    mem = _ell_mem_units(I, J, K, cnt, uniq, has0, dsize)[0]
+    return compute_cost / np.minimum(compute_cost / (mem * 32), peak)
+
+
+def memory_cost_tb_given_tile_sizes_and_poses(tile_sizes, tile_pos_is, template_str, dsize, op):
+    """Memory cost in 32-byte units of ELL tiles at positions tile_pos_is:
+    [non-atomic, atomic] (reference :517-582)."""
+    if template_str != "sparse_template_ell":
+        raise AssertionError("We only support ELL tiles now.")
+    I, J, K = tile_sizes[0][1], tile_sizes[1][1], tile_sizes[2][1]
+    pos_is = np.asarray(tile_pos_is, dtype=np.int64)
+    cnt, uniq, has0 = _tile_unique_cols(op, I, pos_is)
+    return _ell_mem_units(I, J, K, cnt, uniq, has0, dsize)
 
 
 def ELL_non_atomic_coeff(dsize, profile=None):
@@ -230,12 +257,17 @@ def cost_tb_latency_given_ELL_tiles(op, tile_sizes, tile_pos_is, dsize, ori_tot_
 # 1D SDDMM tiles.
 # ---------------------------------------------------------------------------
 
+def sddmm_1d_x(row_nums, col_nums, K, bucket, dsize, peak):
+    """The latency feature of 1D SDDMM buckets: FLOPs over the roofline
+    throughput, bytes = (distinct rows + distinct cols) * K + bucket values."""
+    compute_cost = K * bucket * 2
+    memory_costs = ((np.asarray(row_nums) + np.asarray(col_nums)) * K + bucket) * dsize / 8
+    return compute_cost / np.minimum(compute_cost / memory_costs, peak)
+
+
 def _cost_tb_latency_1D_tiles_given_features(row_nums, col_nums, K, tile_sizes, dsize, profile=None):
     prof = get_profile(profile)
-    compute_cost = K * tile_sizes[0] * 2
-    memory_costs = ((np.asarray(row_nums) + np.asarray(col_nums)) * K + tile_sizes[0]) * dsize / 8
-    throughputs = np.minimum(compute_cost / memory_costs, _peak(dsize, prof))
-    costs = compute_cost / throughputs
+    costs = sddmm_1d_x(row_nums, col_nums, K, tile_sizes[0], dsize, _peak(dsize, prof))
     a, b = prof.sddmm_1d_coeff[16 if dsize == 16 else 32]
     return a * costs + b
 
@@ -265,12 +297,18 @@ def cost_tb_latency_given_1D_tiles(op, tile_sizes, dsize, profile=None):
 # Dense (TC) tiles: measured tables.
 # ---------------------------------------------------------------------------
 
-def get_benchmark_cost_(template_str, tile_sizes, dsize, is_atomic, penalty, profile=None):
+def get_benchmark_cost_(template_str, tile_sizes, dsize, is_atomic, penalty, profile=None, b_rows=None):
     """Table latency of one TC / ELL tile (reference :639-768).  The reference
-    only has FP16 tables; the build's profiles are valid for any dsize."""
+    only has FP16 tables; a measured profile may carry an FP32 table (dsize
+    32) and a B-stationary table (dsize 16 with b_rows, the rows of B, at
+    most LDS_MAX_K)."""
     prof = get_profile(profile)
     key = (tile_sizes[0][1], tile_sizes[1][1], tile_sizes[2][1])
     if template_str == 'TensorCore_template':
+        if dsize == 16 and prof.tc_cost_lds and b_rows is not None and b_rows <= LDS_MAX_K:
+            return prof.tc_cost_lds[key] / penalty
+        if dsize == 32 and prof.tc_cost_f32:
+            return prof.tc_cost_f32[key] / penalty
         tab = prof.tc_cost_atomic if is_atomic else prof.tc_cost
         return tab[key] / penalty
     elif template_str == 'sparse_template_ell':
@@ -282,7 +320,8 @@ def get_benchmark_cost_(template_str, tile_sizes, dsize, is_atomic, penalty, pro
 
 
 def get_benchmark_cost(tile, dsize, is_atomic, penalty, profile=None):
-    return get_benchmark_cost_(get_template_str(tile.op), tile.tile_sizes, dsize, is_atomic, penalty, profile)
+    return get_benchmark_cost_(get_template_str(tile.op), tile.tile_sizes, dsize, is_atomic, penalty, profile,
+                               b_rows=tile.op.idx_lens[2])
 
 
 def cost_tb_latency_given_tile(tile, dsize, ori_tot_nnz, SM_num, is_atomic, penalty, profile=None):

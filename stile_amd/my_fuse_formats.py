@@ -280,11 +280,11 @@ def prepare_inputs_BSDDMM(ori_op, op_type, batch_num, formats, out_shape=None, d
     g1d, tc = [], []
     for fmt, tiles in formats.items():
         if fmt[0] == "1D_sddmm":
+            # tiles of one group may come from different 1D ops (next levels)
             b = fmt[1][0]
-            op = tiles[0].op
-            mid = np.concatenate([op.ori_row_ids_1d[t.tile_pos[0] * b:(t.tile_pos[0] + 1) * b] for t in tiles])
-            jj = np.concatenate([op.ori_col_ids_1d[t.tile_pos[0] * b:(t.tile_pos[0] + 1) * b] for t in tiles])
-            rm = np.concatenate([op.nnz_id_matrix[t.tile_pos[0] * b:(t.tile_pos[0] + 1) * b] for t in tiles])
+            mid = np.concatenate([t.op.ori_row_ids_1d[t.tile_pos[0] * b:(t.tile_pos[0] + 1) * b] for t in tiles])
+            jj = np.concatenate([t.op.ori_col_ids_1d[t.tile_pos[0] * b:(t.tile_pos[0] + 1) * b] for t in tiles])
+            rm = np.concatenate([t.op.nnz_id_matrix[t.tile_pos[0] * b:(t.tile_pos[0] + 1) * b] for t in tiles])
             params = json.loads(fmt[2])
             real = params['ty'] * params['group_size']
             pad = math.ceil(len(mid) / real) * real - len(mid)
@@ -295,6 +295,8 @@ def prepare_inputs_BSDDMM(ori_op, op_type, batch_num, formats, out_shape=None, d
         elif fmt[0] == "TC_sddmm":
             ts = fmt[1]
             op = tiles[0].op
+            if any(t.op is not op for t in tiles):
+                raise AssertionError("TC_sddmm tiles of one group must share their op")
             iv = op.idx_values_list[0]
             # a window or column run cut short by the matrix edge is padded
             # with its last index: no sampled entry points into the padding

@@ -130,3 +130,18 @@ def test_plan_from_saved_pack(cuda, tmp_path):
     p1 = _native.Plan.spmm(api.load_pack(tmp_path / "p.npz"), "float32")
     Bd = _t(B, cuda)
     assert np.array_equal(p0.run_spmm(Bd).cpu().numpy(), p1.run_spmm(Bd).cpu().numpy())
+
+
+def test_tuned_plan_is_correct_and_reports_every_candidate(cuda):
+    A, B = datasets.pruned_layer(seed=4, M=512, K=256, N=96)
+    print("tune_plan pruned 512x256", flush=True)
+    plan, pack, rep = api.tune_plan("spmm", A, "float32", 96)
+    assert set(rep["ms"]) == set(api.tuning_candidates("float32"))
+    assert rep["choice"] == min(rep["ms"], key=rep["ms"].get)
+    _check_fp32(plan.run_spmm(_t(B, cuda)).cpu().numpy(), A, B)
+    X = np.random.default_rng(5).standard_normal((512, 32), dtype=np.float32)
+    Y = np.random.default_rng(6).standard_normal((256, 32), dtype=np.float32)
+    splan, _, srep = api.tune_plan("sddmm", A, "float32", 32)
+    got = splan.run_sddmm(_t(X, cuda), _t(Y, cuda)).cpu().numpy()
+    ref = ops.sddmm_fp64(A, X, Y)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))

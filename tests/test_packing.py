@@ -89,14 +89,20 @@ def test_spmm_pack_matches_oracle_and_reexpands(name, A, opts):
     assert abs(diff).max() == 0 if diff.nnz else True
 
 
-@pytest.mark.parametrize("scale,Kf", [(9, 32), (10, 64), ("mask891", 64)])
+@pytest.mark.parametrize("scale,Kf", [(9, 32), (10, 64), ("mask891", 64), ("gcn11_dsize32", 32)])
 def test_sddmm_pack_matches_oracle(scale, Kf):
+    opts = None
     if scale == "mask891":
         # 891 rows: the last 16-row window is cut short by the matrix edge
         A = datasets.attention_mask(891, band=32, n_random=4, rng=np.random.default_rng(0))
+    elif scale == "gcn11_dsize32":
+        # one 1D group holds tiles of two 1D ops (the first level and a next level)
+        A = datasets.gcn_graph(scale=11, feat=8, seed=1)[0]
+        A.data[:] = 1.0
+        opts = api.SearchOptions(dsize=32)
     else:
         A = datasets.rmat_graph(scale, seed=scale)
-    pack, tiles, op = api.sddmm_pack(A, Kf=Kf)
+    pack, tiles, op = api.sddmm_pack(A, opts, Kf=Kf)
     formats = my_fuse_formats.get_formats_from_selected_tiles(tiles, ['A'], 16)
     seen = np.zeros(A.nnz, bool)
     i1 = it = 0
