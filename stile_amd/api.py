@@ -112,7 +112,7 @@ def sddmm_pack(A, opts=None, Kf=32):
 # rather than across the whole matrix (blocks hold ~10^5 rows).
 # ---------------------------------------------------------------------------
 
-def parallel_map(fn, jobs, workers=None):
+def parallel_map(fn, jobs, workers=None, log=None):
     """fn over jobs in worker processes: forked while this process has not
     touched the GPU, spawned otherwise (a forked copy of an initialised HIP
     runtime is not usable).  Spawned workers re-import the main module, so a
@@ -130,7 +130,21 @@ def parallel_map(fn, jobs, workers=None):
         spawn = False
     ctx = mp.get_context("spawn" if spawn else "fork")
     with ctx.Pool(workers) as pool:
-        return pool.map(fn, jobs, chunksize=1)
+        if log is None:
+            return pool.map(fn, jobs, chunksize=1)
+        import time as _time
+        out, t_last = [None] * len(jobs), _time.time()
+        for done, (i, r) in enumerate(pool.imap_unordered(_indexed, [(fn, i, j) for i, j in enumerate(jobs)]), 1):
+            out[i] = r
+            if _time.time() - t_last > 20 or done == len(jobs):
+                log(f"  {done}/{len(jobs)} jobs done")
+                t_last = _time.time()
+        return out
+
+
+def _indexed(args):
+    fn, i, job = args
+    return i, fn(job)
 
 
 def row_blocks(A, parts):
@@ -200,7 +214,7 @@ def stack_sddmm_packs(packs, cuts, nnz_offs, M, N, nnz):
     return dict(M=int(M), N=int(N), nnz=int(nnz), g1d=g1d, tc=tc)
 
 
-def _blocked_pack(kind, A, opts, width, parts, workers, nnz_per_part):
+def _blocked_pack(kind, A, opts, width, parts, workers, nnz_per_part, log=None):
     A = _csr_of(A)
     if parts is None:
         parts = max(1, int(round(A.nnz / nnz_per_part)))
@@ -208,21 +222,21 @@ def _blocked_pack(kind, A, opts, width, parts, workers, nnz_per_part):
     if len(cuts) == 2:
         return (spmm_pack if kind == "spmm" else sddmm_pack)(A, opts, width)[0]
     blocks = [A[int(a):int(b)] for a, b in zip(cuts[:-1], cuts[1:])]
-    packs = parallel_map(_block_job, [(kind, B, opts, width) for B in blocks], workers)
+    packs = parallel_map(_block_job, [(kind, B, opts, width) for B in blocks], workers, log)
     offs = A.indptr[cuts[:-1]].astype(np.int64)
     if kind == "spmm":
         return stack_spmm_packs(packs, cuts, offs, A.shape[0], A.shape[1])
     return stack_sddmm_packs(packs, cuts, offs, A.shape[0], A.shape[1], A.nnz)
 
 
-def spmm_pack_blocked(A, opts=None, N=32, parts=None, workers=None, nnz_per_part=1 << 21):
+def spmm_pack_blocked(A, opts=None, N=32, parts=None, workers=None, nnz_per_part=1 << 21, log=None):
     """spmm_pack for large A: row blocks of ~nnz_per_part nonzeros searched in
     parallel processes and stacked into one pack (value ids included)."""
-    return _blocked_pack("spmm", A, opts, N, parts, workers, nnz_per_part)
+    return _blocked_pack("spmm", A, opts, N, parts, workers, nnz_per_part, log)
 
 
-def sddmm_pack_blocked(A, opts=None, Kf=32, parts=None, workers=None, nnz_per_part=1 << 21):
-    return _blocked_pack("sddmm", A, opts, Kf, parts, workers, nnz_per_part)
+def sddmm_pack_blocked(A, opts=None, Kf=32, parts=None, workers=None, nnz_per_part=1 << 21, log=None):
+    return _blocked_pack("sddmm", A, opts, Kf, parts, workers, nnz_per_part, log)
 
 
 def options_for(blk_dtype, opts=None):
@@ -332,18 +346,21 @@ def tuning_candidates(blk_dtype, kind="spmm"):
     if my_cost_model.get_profile().name == "mi355x" or os.path.exists(my_cost_model.PROFILE_PATH):
         cands["mi355x"] = SearchOptions(dsize=dsize, profile="mi355x")
     cands["a100"] = SearchOptions(dsize=dsize, profile="a100")
-    cands["only_tc"] = SearchOptions(dsize=dsize, only_TC=True)
+    if kind == "spmm":
+        # (an all-TC SDDMM covers every output with 16 x 16 blocks: a slow
+        # search that has not beaten the others on any workload measured)
+        cands["only_tc"] = SearchOptions(dsize=dsize, only_TC=True)
     cands["only_sparse"] = SearchOptions(dsize=dsize, only_ELL=True)
     return cands
 
 
-def _pack_any(kind, mats, opts, width, workers):
+def _pack_any(kind, mats, opts, width, workers, log=None):
     if isinstance(mats, (list, tuple)):
         job = [(kind, A, opts, width) for A in mats]
-        return parallel_map(_block_job, job, workers)
+        return parallel_map(_block_job, job, workers, log)
     if kind == "spmm":
-        return spmm_pack_blocked(mats, opts, width, workers=workers)
-    return sddmm_pack_blocked(mats, opts, width, workers=workers)
+        return spmm_pack_blocked(mats, opts, width, workers=workers, log=log)
+    return sddmm_pack_blocked(mats, opts, width, workers=workers, log=log)
 
 
 def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps=20, workers=None, log=None):
@@ -361,7 +378,7 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
     packs, search_s = {}, {}
     for name, opts in cands.items():          # every search before the GPU is used here
         t0 = _time.perf_counter()
-        packs[name] = _pack_any(kind, mats, opts, width, workers)
+        packs[name] = _pack_any(kind, mats, opts, width, workers, log)
         search_s[name] = round(_time.perf_counter() - t0, 3)
         if log:
             log(f"tune {kind}: searched {name} in {search_s[name]} s")
