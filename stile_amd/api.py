@@ -283,12 +283,8 @@ def build_sddmm_plan_batched(mats, opts=None, Kf=32):
 _GROUP_LISTS = ("ell", "tc", "g1d")
 
 
-def save_pack(path, pack):
-    """Write an SpMM or SDDMM pack (the dicts of spmm_pack / sddmm_pack /
-    the blocked variants) to an .npz file: array fields as arrays, scalar
-    fields and group counts as a JSON header."""
-    import json
-    arrays, meta = {}, {"scalars": {}, "groups": {}}
+def _pack_to_arrays(pack, prefix, arrays):
+    meta = {"scalars": {}, "groups": {}}
     for k, v in pack.items():
         if k in _GROUP_LISTS:
             meta["groups"][k] = []
@@ -298,7 +294,7 @@ def save_pack(path, pack):
                     if x is None:
                         gm[f] = None
                     elif isinstance(x, np.ndarray):
-                        arrays[f"{k}{i}.{f}"] = x
+                        arrays[f"{prefix}{k}{i}.{f}"] = x
                         gm[f] = "array"
                     else:
                         gm[f] = x.item() if isinstance(x, np.generic) else x
@@ -307,26 +303,43 @@ def save_pack(path, pack):
             meta["order"] = [list(o) for o in v]
         else:
             meta["scalars"][k] = int(v)
+    return meta
+
+
+def _pack_from_arrays(meta, prefix, z):
+    pack = dict(meta["scalars"])
+    for k, groups in meta["groups"].items():
+        pack[k] = []
+        for i, gm in enumerate(groups):
+            pack[k].append({f: (np.array(z[f"{prefix}{k}{i}.{f}"]) if x == "array" else x) for f, x in gm.items()})
+    if "order" in meta:
+        pack["order"] = [tuple(o) for o in meta["order"]]
+    return pack
+
+
+def save_pack(path, pack):
+    """Write an SpMM or SDDMM pack (the dicts of spmm_pack / sddmm_pack /
+    the blocked variants), or a list of them (a batched plan's packs), to an
+    .npz file: array fields as arrays, scalar fields and group counts as a
+    JSON header."""
+    import json
+    arrays = {}
+    if isinstance(pack, (list, tuple)):
+        meta = {"list": [_pack_to_arrays(p, f"p{i}/", arrays) for i, p in enumerate(pack)]}
+    else:
+        meta = _pack_to_arrays(pack, "", arrays)
     arrays["__meta__"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez(path, **arrays)
 
 
 def load_pack(path):
-    """Read a pack written by save_pack."""
+    """Read a pack (or list of packs) written by save_pack."""
     import json
     with np.load(path, allow_pickle=False) as z:
         meta = json.loads(bytes(z["__meta__"]).decode())
-        pack = dict(meta["scalars"])
-        for k, groups in meta["groups"].items():
-            pack[k] = []
-            for i, gm in enumerate(groups):
-                g = {}
-                for f, x in gm.items():
-                    g[f] = np.array(z[f"{k}{i}.{f}"]) if x == "array" else x
-                pack[k].append(g)
-        if "order" in meta:
-            pack["order"] = [tuple(o) for o in meta["order"]]
-    return pack
+        if "list" in meta:
+            return [_pack_from_arrays(m, f"p{i}/", z) for i, m in enumerate(meta["list"])]
+        return _pack_from_arrays(meta, "", z)
 
 
 # ---------------------------------------------------------------------------
@@ -363,17 +376,24 @@ def _pack_any(kind, mats, opts, width, workers, log=None):
     return sddmm_pack_blocked(mats, opts, width, workers=workers, log=log)
 
 
-def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps=20, workers=None, log=None):
+def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps=20, workers=None, log=None,
+              cache=None):
     """Build the plan of `mats` (one matrix, or a list for a batched plan)
     under each candidate search setting and return (fastest plan, its pack,
     report).  kind: 'spmm' (width = N) or 'sddmm' (width = K_feat, FP32
     blocks).  report: {"choice": name, "ms": {name: ms per run},
-    "search_s": {name: seconds}}."""
+    "search_s": {name: seconds}}.
+    cache: an .npz path; if it exists its pack is used as is (no search, no
+    timing: report {"choice": "cached"}), else the chosen pack is saved there."""
     import time as _time
     import torch
     from . import _native
     if kind == "sddmm":
         blk_dtype = "float32"
+    if cache is not None and os.path.exists(cache):
+        pack = load_pack(cache)
+        plan = _native.Plan.spmm(pack, blk_dtype) if kind == "spmm" else _native.Plan.sddmm(pack, "float32")
+        return plan, pack, {"choice": "cached", "cache": str(cache)}
     cands = candidates or tuning_candidates(blk_dtype, kind)
     packs, search_s = {}, {}
     for name, opts in cands.items():          # every search before the GPU is used here
@@ -423,6 +443,9 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
             best, best_ms, choice = plan, ms, name
         else:
             plan.close()
+    if cache is not None:
+        os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+        save_pack(cache, packs[choice])
     return best, packs[choice], {"choice": choice, "ms": times, "search_s": search_s}
 
 

@@ -16,6 +16,7 @@ many B (my_fuse_formats.py:1296-1428); this pipeline is the batched-attention
 workload built from its SDDMM and SpMM.
 """
 import math
+import os
 import numpy as np
 import scipy.sparse
 import torch
@@ -40,7 +41,7 @@ class BatchedSparseAttention(object):
     """Sparse attention over a list of CSR masks (one per instance)."""
 
     def __init__(self, masks, head_dim, blk_dtype="bfloat16", opts=None, device="cuda", workers=None,
-                 tune=False, log=None):
+                 tune=False, log=None, cache_dir=None):
         self.masks = [scipy.sparse.csr_matrix(m) for m in masks]
         for m in self.masks:
             m.sort_indices()
@@ -53,8 +54,11 @@ class BatchedSparseAttention(object):
             # one search setting for all instances, chosen by measured latency
             ones = [scipy.sparse.csr_matrix((np.ones(m.nnz, np.float32), m.indices, m.indptr), shape=m.shape)
                     for m in self.masks]
-            self.spmm_plan, _, rep = api.tune_plan("spmm", ones, blk_dtype, head_dim, workers=workers, log=log)
-            self.sddmm_plan, _, srep = api.tune_plan("sddmm", ones, "float32", head_dim, workers=workers, log=log)
+            cp = (lambda name: None if cache_dir is None else os.path.join(cache_dir, name))
+            self.spmm_plan, _, rep = api.tune_plan("spmm", ones, blk_dtype, head_dim, workers=workers, log=log,
+                                                   cache=cp("spmm.npz"))
+            self.sddmm_plan, _, srep = api.tune_plan("sddmm", ones, "float32", head_dim, workers=workers, log=log,
+                                                     cache=cp("sddmm.npz"))
             self.tuning = {"spmm": rep, "sddmm": srep}
         else:
             spmm_packs = build_packs("spmm", self.masks, api.options_for(blk_dtype, opts), workers)

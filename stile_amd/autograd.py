@@ -17,6 +17,7 @@ the config the GCN workload (forward A.(HW), backward A^T SpMM + SDDMM for
 dA) needs.
 """
 import copy
+import os
 
 import numpy as np
 import scipy.sparse
@@ -50,7 +51,7 @@ class SparseOperand(object):
     accumulation).  opts: api.SearchOptions for the tile search."""
 
     def __init__(self, A, blk_dtype="float32", opts=None, device="cuda", N=32, workers=None,
-                 nnz_per_part=1 << 21, tune=False, log=None):
+                 nnz_per_part=1 << 21, tune=False, log=None, cache_dir=None):
         A = scipy.sparse.csr_matrix(A, dtype=np.float32)
         A.sort_indices()
         self.A = A
@@ -68,15 +69,19 @@ class SparseOperand(object):
         self.tuning = None
         if tune:
             # plans chosen by measured latency at width N (api.tune_plan)
-            self.plan, pack, rep = api.tune_plan("spmm", ones, blk_dtype, N, workers=workers, log=log)
+            cp = (lambda name: None if cache_dir is None else os.path.join(cache_dir, name))
+            self.plan, pack, rep = api.tune_plan("spmm", ones, blk_dtype, N, workers=workers, log=log,
+                                                 cache=cp("spmm.npz"))
             srep = None
             if self.symmetric:
                 tpack = copy.deepcopy(pack)
             else:
                 tones = scipy.sparse.csr_matrix((np.ones(T.nnz, np.float32), T.indices, T.indptr), shape=T.shape)
-                plan_t, tpack, _ = api.tune_plan("spmm", tones, blk_dtype, N, workers=workers, log=log)
+                plan_t, tpack, _ = api.tune_plan("spmm", tones, blk_dtype, N, workers=workers, log=log,
+                                                 cache=cp("spmm_t.npz"))
                 plan_t.close()
-            self.sddmm_plan, _, srep = api.tune_plan("sddmm", ones, "float32", N, workers=workers, log=log)
+            self.sddmm_plan, _, srep = api.tune_plan("sddmm", ones, "float32", N, workers=workers, log=log,
+                                                     cache=cp("sddmm.npz"))
             self.tuning = {"spmm": rep, "sddmm": srep}
         else:
             opts = api.options_for(blk_dtype, opts)

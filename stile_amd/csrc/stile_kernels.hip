@@ -68,12 +68,22 @@ __device__ __forceinline__ float4 load_b4(const float* __restrict__ B,
   return r;
 }
 
+// NT: non-temporal store (streamed past the caches; for rows nothing reads
+// back soon).
+template <bool NT = false>
 __device__ __forceinline__ void store_c4(float* __restrict__ C, int64_t ldc,
                                          int row, int col, int N, bool vec,
                                          float4 v) {
   float* p = C + (int64_t)row * ldc + col;
   if (vec) {
-    if (col < N) st4(p, v);
+    if (col < N) {
+      if constexpr (NT) {
+        typedef float f32v4 __attribute__((ext_vector_type(4)));
+        __builtin_nontemporal_store(f32v4{v.x, v.y, v.z, v.w}, reinterpret_cast<f32v4*>(p));
+      } else {
+        st4(p, v);
+      }
+    }
     return;
   }
   if (col < N) p[0] = v.x;
@@ -314,6 +324,7 @@ hipError_t launch_csr_sddmm_f32(int M, int nnz, int Kf, const int32_t* indptr,
 #define STILE_SPARSE_DEPTH 8
 #endif
 
+
 // DEPTH 8 keeps eight gathers in flight with the next ids prefetched; DEPTH 4
 // (four at a time) is for kernels whose register budget is tight.
 template <int DEPTH>
@@ -413,12 +424,13 @@ __device__ __forceinline__ SlotRows load_rows(const SpmmPlanDev& p, int slot0) {
 // Store the dense accumulators of the lane group's 4 rows (acc[t][i] is row
 // 4g+i, column col+t).  The sparse parts of these rows, if any, are added
 // afterwards by hyb_spmm_sparse_kernel.
+template <bool NT = false>
 __device__ __forceinline__ void store_window(float* __restrict__ C, int64_t ldc, int N, int col, bool vec,
                                              const SlotRows& sr, const f32x4 (&acc)[4]) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
     if (sr.row[i] >= 0)
-      store_c4(C, ldc, sr.row[i], col, N, vec, make_float4(acc[0][i], acc[1][i], acc[2][i], acc[3][i]));
+      store_c4<NT>(C, ldc, sr.row[i], col, N, vec, make_float4(acc[0][i], acc[1][i], acc[2][i], acc[3][i]));
 }
 
 // Slot of a dense window with a sparse part: C[row] += first kSplitNnz
@@ -643,8 +655,10 @@ __device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
 }
 
 __device__ __forceinline__ uint32_t half_sel(uint32_t x, uint32_t y, int hi) {
-  // (x.lo, y.lo) or (x.hi, y.hi) as one packed pair
-  return hi ? ((x >> 16) | (y & 0xffff0000u)) : ((x & 0xffffu) | (y << 16));
+  // (x.lo, y.lo) or (x.hi, y.hi) as one packed pair: one v_perm_b32, whose
+  // selector bytes 0-3 pick bytes of its second operand (x), 4-7 of its
+  // first (y)
+  return __builtin_amdgcn_perm(y, x, hi ? 0x07060302u : 0x05040100u);
 }
 
 constexpr int kLdsThreads = 1024;
@@ -700,7 +714,6 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
                                                                         int64_t ldc, int nC, int wpg, bool vec) {
   extern __shared__ uint2 bsh[];  // [K][16]: 4 BF16 per entry
   constexpr int kWaves = kLdsThreads / 64;
-  constexpr int kDepth = 1;  // chunks fetched ahead per step
   const int chunk = blockIdx.x % nC;
   const int grp = blockIdx.x / nC;
   const int n0 = chunk * 64;
@@ -742,41 +755,46 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   if (w_beg >= w_end) return;
   const int c_beg = p.tc_ptr[w_beg], c_end = p.tc_ptr[w_end];
 
-  f32x4 acc[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // The operands of chunk c + 2 are requested while chunk c is multiplied
+  // (indices clamped to the wave's last chunk, so every load is
+  // unconditional).  Three register sets rotate through a loop unrolled by
+  // three, so no set is copied (a copy would wait for the newest load).
+  // Every window of the run holds at least one chunk, so at most one window
+  // ends before each chunk.
+  const int last = c_end - 1;
   int w = w_beg;
   int w_next = p.tc_ptr[w + 1];
   SlotRows sr = load_rows(p, w * 16 + g * 4);
-  ChunkRegs cur[kDepth];
+  f32x4 acc[4];
 #pragma unroll
-  for (int u = 0; u < kDepth; ++u)
-    if (c_beg + u < c_end) cur[u] = load_chunk_regs(p, c_beg + u, g, lane);
-  for (int c = c_beg; c < c_end; c += kDepth) {
-    ChunkRegs nxt[kDepth];
+  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  ChunkRegs r0 = load_chunk_regs(p, c_beg, g, lane);
+  ChunkRegs r1 = load_chunk_regs(p, min(c_beg + 1, last), g, lane);
+  ChunkRegs r2;
+  int c = c_beg;
+  auto step = [&](const ChunkRegs& use, ChunkRegs& fill) {
+    fill = load_chunk_regs(p, min(c + 2, last), g, lane);
+    if (c >= w_next) {  // window w is complete: store it
+      const SlotRows sr_next = load_rows(p, (w + 1) * 16 + g * 4);
+      const int w_next2 = p.tc_ptr[w + 2];
+      store_window<true>(C, ldc, N, col, vec, sr, acc);
 #pragma unroll
-    for (int u = 0; u < kDepth; ++u)
-      if (c + kDepth + u < c_end) nxt[u] = load_chunk_regs(p, c + kDepth + u, g, lane);
-#pragma unroll
-    for (int u = 0; u < kDepth; ++u) {
-      const int cc = c + u;
-      if (cc >= c_end) break;
-      while (cc >= w_next) {  // window w is complete: store it
-        const SlotRows sr_next = load_rows(p, (w + 1) * 16 + g * 4);
-        const int w_next2 = p.tc_ptr[w + 2];
-        store_window(C, ldc, N, col, vec, sr, acc);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        sr = sr_next;
-        ++w;
-        w_next = w_next2;
-      }
-      lds_chunk_mfma(bsh, j, cur[u].ka, cur[u].kb, cur[u].a, acc);
+      for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+      sr = sr_next;
+      ++w;
+      w_next = w_next2;
     }
-#pragma unroll
-    for (int u = 0; u < kDepth; ++u) cur[u] = nxt[u];
+    lds_chunk_mfma(bsh, j, use.ka, use.kb, use.a, acc);
+    ++c;
+  };
+  while (c + 3 <= c_end) {
+    step(r0, r2);
+    step(r1, r0);
+    step(r2, r1);
   }
-  store_window(C, ldc, N, col, vec, sr, acc);
+  if (c < c_end) step(r0, r2);
+  if (c < c_end) step(r1, r0);
+  store_window<true>(C, ldc, N, col, vec, sr, acc);
 }
 
 // Remaining pieces of long sparse rows: lane group g of the wave takes
