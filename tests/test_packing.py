@@ -204,6 +204,20 @@ def test_pack_save_load_roundtrip(tmp_path, kind):
             assert back[k] == v
 
 
+def test_pack_list_save_load_roundtrip(tmp_path):
+    mats = [datasets.attention_mask(L, band=16, n_random=2, rng=np.random.default_rng(L)) for L in (40, 70)]
+    packs = [api.spmm_pack(m)[0] for m in mats]
+    api.save_pack(tmp_path / "l.npz", packs)
+    back = api.load_pack(tmp_path / "l.npz")
+    assert len(back) == 2
+    for p0, p1 in zip(packs, back):
+        for k in ("ell", "tc"):
+            for g0, g1 in zip(p0[k], p1[k]):
+                for f in g0:
+                    if isinstance(g0[f], np.ndarray):
+                        assert np.array_equal(g0[f], g1[f])
+
+
 def test_load_snap_graph_relabels_dedupes_and_pads(tmp_path):
     from stile_amd.gen_formats_v2 import load_snap_graph
     p = tmp_path / "g.txt"
