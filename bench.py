@@ -15,8 +15,9 @@ Other workloads (--workload), each printed as its own JSON line:
                 normalised), H 1M x 128, W 128 x 128 FP32: forward A.(HW),
                 backward A^T.dY and SDDMM dA (one step = forward + backward)
   attention     64 sparse-attention instances, L ~ U[512, 8192], d = 64,
-                band 128 + 16 random columns: SDDMM(QK^T), row softmax,
-                SpMM with V, replayed as one HIP graph (one step = one replay)
+                band 128 + 16 random columns: softmax(QK^T / 8 on the mask) V
+                in one fused kernel, replayed as a HIP graph (one step = one
+                replay); the unfused SDDMM -> softmax -> SpMM graph beside it
   all           every workload above, one line each
 
 Each line carries GFLOP/s (2 nnz N per sparse product), the HBM roofline of
@@ -152,6 +153,14 @@ def _spmm_comparators(A, Bd, steps, warmup):
     return ms_csr, ms_vendor
 
 
+def cache_path(args, name):
+    return None if args.plan_cache is None else os.path.join(args.plan_cache, name + ".npz")
+
+
+def cache_dir(args, name):
+    return None if args.plan_cache is None else os.path.join(args.plan_cache, name)
+
+
 def progress(msg):
     """Progress of the long set-up steps (search, tuning) on stderr."""
     print(msg, file=sys.stderr, flush=True)
@@ -174,7 +183,8 @@ def run_pruned_layer(args, dev):
     A, B = datasets.pruned_layer(seed=0, N=N)
     M, K = A.shape
     t0 = time.perf_counter()
-    plan, _, tuning = api.tune_plan("spmm", A, "bfloat16", N, log=progress)
+    plan, _, tuning = api.tune_plan("spmm", A, "bfloat16", N, log=progress,
+                                    cache=cache_path(args, f"pruned_layer_n{N}"))
     t_plan = time.perf_counter() - t0
     info = plan.info()
     Bd = torch.from_numpy(B).to(dev)
@@ -218,7 +228,7 @@ def run_uniform_spmm(args, dev):
     M, K = A.shape
     N = B.shape[1]
     t0 = time.perf_counter()
-    plan, _, tuning = api.tune_plan("spmm", A, "float32", N, log=progress)
+    plan, _, tuning = api.tune_plan("spmm", A, "float32", N, log=progress, cache=cache_path(args, "uniform_spmm"))
     t_plan = time.perf_counter() - t0
     Bd = torch.from_numpy(B).to(dev)
     C = torch.empty((M, N), device=dev, dtype=torch.float32)
@@ -248,7 +258,7 @@ def run_rmat_sddmm(args, dev):
     M, Nc = A.shape
     Kf = X.shape[1]
     t0 = time.perf_counter()
-    plan, _, tuning = api.tune_plan("sddmm", A, "float32", Kf, log=progress)
+    plan, _, tuning = api.tune_plan("sddmm", A, "float32", Kf, log=progress, cache=cache_path(args, "rmat_sddmm"))
     t_plan = time.perf_counter() - t0
     Xd, Yd = torch.from_numpy(X).to(dev), torch.from_numpy(Y).to(dev)
     out_d = torch.empty(A.nnz, device=dev, dtype=torch.float32)
@@ -290,7 +300,8 @@ def run_gcn(args, dev):
     F = H.shape[1]
     t0 = time.perf_counter()
     progress(f"gcn: graph scale {args.gcn_scale}, nnz {A.nnz}")
-    op = SparseOperand(A, blk_dtype="float32", device=dev, N=F, tune=True, log=progress)
+    op = SparseOperand(A, blk_dtype="float32", device=dev, N=F, tune=True, log=progress,
+                       cache_dir=cache_dir(args, f"gcn{args.gcn_scale}"))
     t_plan = time.perf_counter() - t0
     Hd = torch.from_numpy(H).to(dev).requires_grad_(True)
     Wd = torch.from_numpy(W).to(dev).requires_grad_(True)
@@ -350,7 +361,8 @@ def run_attention(args, dev):
     masks = datasets.attention_masks(count=64)
     d = 64
     t0 = time.perf_counter()
-    att = BatchedSparseAttention(masks, d, blk_dtype="bfloat16", device=dev, tune=True, log=progress)
+    att = BatchedSparseAttention(masks, d, blk_dtype="bfloat16", device=dev, tune=True, log=progress,
+                                 cache_dir=cache_dir(args, "attention"))
     t_plan = time.perf_counter() - t0
     n = int(att.row_off[-1])
     g = torch.Generator(device=dev)
@@ -359,19 +371,31 @@ def run_attention(args, dev):
     eager = att.forward(Q, K, V).clone()
     att.capture(Q, K, V)
     ms = time_launches(att.replay, max(args.steps // 4, 10), 3)
-    q, k, v = Q.float(), K.float(), V.float()
-    ms_sd = time_launches(lambda: att.sddmm_plan.run_sddmm(q, k, att.scores), 20, 3)
-    ms_sp = time_launches(lambda: att.spmm_plan.run_spmm(v, att.gout), 20, 3)
+    o2 = torch.empty((n, d), device=dev, dtype=torch.float32)
+    ms_fused = time_launches(lambda: att.att_plan.run_attention(Q, K, V, att.scale, out=o2),
+                             max(args.steps // 4, 10), 3)
+    # the unfused pipeline (SDDMM -> row softmax -> value refresh -> SpMM, one
+    # graph) with the default search, for comparison
+    progress("attention: unfused pipeline for comparison")
+    un = BatchedSparseAttention(masks, d, blk_dtype="bfloat16", device=dev, fused=False)
+    un.capture(Q, K, V)
+    ms_unfused = time_launches(un.replay, max(args.steps // 10, 5), 2)
+    unfused_equal = float((un.gout - att.gout).abs().max())
     flops = 2 * 2 * att.nnz * d
-    out = line("batched sparse attention GFLOP/s (SDDMM + SpMM, 2 x 2*nnz*d per graph replay)",
-               flops / (ms * 1e-3) / 1e9, "GFLOP/s", ms, args, "bf16 inputs, f32 scores/softmax, bf16 MFMA blocks",
+    # algorithmic bytes of the fused kernel: Q, K, V (BF16) and out (FP32)
+    # once, 4 bytes of column index per mask entry, window row ids
+    alg = 3 * n * d * 2 + n * d * 4 + att.nnz * 4 + 4 * n
+    out = line("batched sparse attention GFLOP/s (QK^T and PV on the masks, 2 x 2*nnz*d per graph replay)",
+               flops / (ms * 1e-3) / 1e9, "GFLOP/s", ms, args,
+               "bf16 Q/K/V and probabilities, f32 scores/softmax/accumulation",
                {"workload": f"64 attention instances, L~U[512,8192] (sum {n}), d={d}, band 128 + 16 random "
-                            f"(nnz={att.nnz}), one HIP graph", "instances": 64, "rows": n, "nnz": int(att.nnz),
-                "parallelism": "single GPU"},
-               roofline=roofline(spmm_alg_bytes(att.nnz, n, n, d), ms_sp),
-               kernels_ms={"sddmm": round(ms_sd, 4), "spmm": round(ms_sp, 4)},
-               plan={"spmm": att.spmm_plan.info(), "sddmm": att.sddmm_plan.info(),
-                     "search_tune_and_plan_s": round(t_plan, 3), "tuning": att.tuning})
+                            f"(nnz={att.nnz}), fused kernel in one HIP graph", "instances": 64, "rows": n,
+                "nnz": int(att.nnz), "parallelism": "single GPU"},
+               roofline=roofline(alg, ms_fused),
+               kernels_ms={"fused_attention": round(ms_fused, 4), "unfused_graph": round(ms_unfused, 4)},
+               unfused_max_abs_diff=unfused_equal,
+               plan={"attention": att.att_plan.info(), "search_tune_and_plan_s": round(t_plan, 3),
+                     "tuning": att.tuning})
     if not args.no_cpu:
         att.replay()
         torch.cuda.synchronize()
@@ -426,6 +450,8 @@ def main():
     ap.add_argument("--workload", default="pruned_layer", choices=sorted(WORKLOADS) + ["all"])
     ap.add_argument("--gcn-scale", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plan-cache", default=None,
+                    help="directory of tuned packs: used if present (no search), written otherwise")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the tile path would shard "

@@ -232,13 +232,30 @@ int stile_plan_create_sddmm_batched(const StileSddmmPack* packs, int32_t count,
  * step (the SDDMM backward multiplies by the incoming gradient). */
 int stile_plan_set_values(StilePlan* plan, const float* vals, int64_t nnz, void* stream);
 
+/* Fused sparse attention (no counterpart in the reference, which has these
+ * masks only as SpMM patterns, gen_formats_v2.py:2926-3003, and batches one
+ * pattern over many B, my_fuse_formats.py:1296-1428).  The plan is built like
+ * stile_plan_create_spmm_batched from `count` square mask packs (block
+ * diagonal; every packed slot is a mask entry) with BF16 blocks, plus per-chunk
+ * entry bitmaps; it also runs as an SpMM plan. */
+int stile_plan_create_attention(const StileSpmmPack* packs, int32_t count, StilePlan** out);
+
+/* out[r] = sum_c softmax_c(scale * <Q[r], K[c]>) V[c] over the mask entries
+ * (r, c) of the plan, for every row r of the stacked instances; rows without
+ * entries get 0.  Q, K, V: BF16 device arrays (rows x d, leading dimension ld,
+ * a multiple of 8, 16-byte aligned); out: FP32 (rows x d, leading dimension
+ * ldo, a multiple of 4); d must be 64; scale > 0.  Launches one kernel on
+ * `stream`, allocates nothing (capturable). */
+int stile_attention_run(const StilePlan* plan, const void* Q, const void* K, const void* V, int64_t ld, int32_t d,
+                        float scale, float* out, int64_t ldo, void* stream);
+
 int stile_plan_info(const StilePlan* plan, StilePlanInfo* info);
 int stile_plan_destroy(StilePlan* plan);
 
 /* Message of the last failure on this thread ("" if none). */
 const char* stile_last_error(void);
 /* ABI version, bumped on any layout change of the structs above. */
-int32_t stile_abi_version(void);  /* 2 */
+int32_t stile_abi_version(void);  /* 3 */
 
 #ifdef __cplusplus
 }

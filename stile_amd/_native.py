@@ -91,6 +91,11 @@ _SIGNATURES = {
                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                        ctypes.c_void_p, ctypes.c_void_p]),
     "stile_plan_set_values": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+    "stile_plan_create_attention": (ctypes.c_int, [ctypes.POINTER(StileSpmmPack), ctypes.c_int32,
+                                                   ctypes.POINTER(ctypes.c_void_p)]),
+    "stile_attention_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_int64, ctypes.c_int32, ctypes.c_float, ctypes.c_void_p,
+                                           ctypes.c_int64, ctypes.c_void_p]),
     "stile_plan_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(StilePlanInfo)]),
     "stile_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "stile_last_error": (ctypes.c_char_p, []),
@@ -295,6 +300,37 @@ class Plan:
         _check(lib.stile_plan_create_spmm_batched(structs, len(packs), blk_dtype_code(blk_dtype),
                                                   ctypes.byref(h)))
         return cls(h, "spmm", sum(p["M"] for p in packs), sum(p["K"] for p in packs))
+
+    @classmethod
+    def attention(cls, packs):
+        """Fused-attention plan over square mask packs (one per instance,
+        block-diagonal); also runs as a BF16-block SpMM plan."""
+        lib = load()
+        packs = packs if isinstance(packs, (list, tuple)) else [packs]
+        arrays = _PackArrays()
+        structs = (StileSpmmPack * len(packs))(*[_spmm_pack_struct(p, arrays) for p in packs])
+        h = ctypes.c_void_p()
+        _check(lib.stile_plan_create_attention(structs, len(packs), ctypes.byref(h)))
+        plan = cls(h, "spmm", sum(p["M"] for p in packs), sum(p["K"] for p in packs))
+        plan.attention_plan = True
+        return plan
+
+    def run_attention(self, Q, K, V, scale, out=None):
+        """out = softmax(scale * Q K^T on the masks) V; Q, K, V BF16 (rows x 64,
+        rows = the stacked instances), out FP32."""
+        import torch
+        if not getattr(self, "attention_plan", False):
+            raise StileError("not an attention plan")
+        _dev_check(Q, K, V)
+        for t in (Q, K, V):
+            if t.dtype != torch.bfloat16 or t.shape != (self.M, 64):
+                raise StileError("Q, K, V must be bfloat16 of shape (rows, 64)")
+        if not (Q.stride() == K.stride() == V.stride()) or Q.stride(1) != 1:
+            Q, K, V = Q.contiguous(), K.contiguous(), V.contiguous()
+        o = out if out is not None else torch.empty((self.M, 64), device=Q.device, dtype=torch.float32)
+        _check(load().stile_attention_run(self._h, Q.data_ptr(), K.data_ptr(), V.data_ptr(), Q.stride(0), 64,
+                                          float(scale), o.data_ptr(), o.stride(0), _stream(Q)))
+        return o
 
     @classmethod
     def sddmm(cls, packs, blk_dtype="float32"):

@@ -37,11 +37,58 @@ def build_packs(kind, mats, opts=None, workers=None):
     return api.parallel_map(_pack_one, [(kind, A, opts) for A in mats], workers)
 
 
+def _tune_attention(ones, opts, workers, log, cache_dir, device):
+    """Fused-attention plan: the search setting (see api.tuning_candidates)
+    whose plan runs the fused kernel fastest on random Q, K, V."""
+    import time as _time
+    cache = None if cache_dir is None else os.path.join(cache_dir, "attention_fused.npz")
+    if cache is not None and os.path.exists(cache):
+        return _native.Plan.attention(api.load_pack(cache)), {"choice": "cached", "cache": cache}
+    cands = api.tuning_candidates("bfloat16")
+    packs, search_s = {}, {}
+    for name, o in cands.items():
+        t0 = _time.perf_counter()
+        packs[name] = build_packs("spmm", ones, o, workers)
+        search_s[name] = round(_time.perf_counter() - t0, 3)
+        if log:
+            log(f"tune attention: searched {name} in {search_s[name]} s")
+    n = sum(m.shape[0] for m in ones)
+    g = torch.Generator(device=device)
+    g.manual_seed(0)
+    Q, K, V = (torch.randn(n, 64, device=device, generator=g).to(torch.bfloat16) for _ in range(3))
+    best, best_ms, times, choice = None, float("inf"), {}, None
+    for name, pk in packs.items():
+        plan = _native.Plan.attention(pk)
+        for _ in range(3):
+            plan.run_attention(Q, K, V, 0.125)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(10):
+            plan.run_attention(Q, K, V, 0.125)
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / 10
+        times[name] = round(ms, 5)
+        if log:
+            log(f"tune attention {name}: {ms * 1e3:.1f} us")
+        if ms < best_ms:
+            if best is not None:
+                best.close()
+            best, best_ms, choice = plan, ms, name
+        else:
+            plan.close()
+    if cache is not None:
+        os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+        api.save_pack(cache, packs[choice])
+    return best, {"choice": choice, "ms": times, "search_s": search_s}
+
+
 class BatchedSparseAttention(object):
     """Sparse attention over a list of CSR masks (one per instance)."""
 
     def __init__(self, masks, head_dim, blk_dtype="bfloat16", opts=None, device="cuda", workers=None,
-                 tune=False, log=None, cache_dir=None):
+                 tune=False, log=None, cache_dir=None, fused=None):
         self.masks = [scipy.sparse.csr_matrix(m) for m in masks]
         for m in self.masks:
             m.sort_indices()
@@ -50,7 +97,20 @@ class BatchedSparseAttention(object):
         self.device = torch.device(device)
         self.row_off = np.concatenate([[0], np.cumsum(self.lens)])
         self.tuning = None
-        if tune:
+        # fused: one kernel (stile_attention_run) for head dimension 64;
+        # otherwise SDDMM -> row softmax -> value refresh -> SpMM
+        self.fused = (head_dim == 64) if fused is None else fused
+        if self.fused:
+            if head_dim != 64:
+                raise ValueError("the fused kernel supports head dimension 64")
+            ones = [scipy.sparse.csr_matrix((np.ones(m.nnz, np.float32), m.indices, m.indptr), shape=m.shape)
+                    for m in self.masks]
+            if tune:
+                self.att_plan, self.tuning = _tune_attention(ones, opts, workers, log, cache_dir, device)
+            else:
+                packs = build_packs("spmm", self.masks, api.options_for(blk_dtype, opts), workers)
+                self.att_plan = _native.Plan.attention(packs)
+        elif tune:
             # one search setting for all instances, chosen by measured latency
             ones = [scipy.sparse.csr_matrix((np.ones(m.nnz, np.float32), m.indices, m.indptr), shape=m.shape)
                     for m in self.masks]
@@ -71,12 +131,17 @@ class BatchedSparseAttention(object):
         self.indptr = torch.from_numpy(big.indptr.astype(np.int32)).to(self.device)
         self.indices = big.indices.astype(np.int32)
         self.scale = 1.0 / math.sqrt(head_dim)
-        self.scores = torch.empty(self.nnz, device=self.device, dtype=torch.float32)
-        self.probs = torch.empty(self.nnz, device=self.device, dtype=torch.float32)
+        if not self.fused:
+            self.scores = torch.empty(self.nnz, device=self.device, dtype=torch.float32)
+            self.probs = torch.empty(self.nnz, device=self.device, dtype=torch.float32)
         self.graph = None
 
     def forward(self, Q, K, V, out=None):
-        """Q, K, V: (sum L_b) x d stacked (any float dtype; computed in FP32)."""
+        """Q, K, V: (sum L_b) x d stacked.  Fused: BF16 operands (others are
+        converted), one kernel; unfused: computed in FP32."""
+        if self.fused:
+            q, k, v = (t if t.dtype == torch.bfloat16 else t.to(torch.bfloat16) for t in (Q, K, V))
+            return self.att_plan.run_attention(q, k, v, self.scale, out=out)
         q = Q.float().contiguous()
         k = K.float().contiguous()
         v = V.float().contiguous()

@@ -55,6 +55,7 @@ struct SpmmPlanDev {
   const float* res_vals;
   int32_t n_mix;             // slots of dense windows with a sparse part
   const int32_t* mix_slot;   // [n_mix]
+  const uint32_t* tc_mask;   // [nchunks][16] entry bits (attention plans), else nullptr
 };
 
 // Device view of a hybrid SDDMM plan.
@@ -99,6 +100,8 @@ hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,
 hipError_t launch_set_values(const SpmmPlanDev& p, const float* vals, const int32_t* tc_ids, int64_t n_tc,
                              const int32_t* res_ids, int64_t n_res, hipStream_t s);
 
+hipError_t launch_attention(const SpmmPlanDev& p, const void* Q, const void* K, const void* V, int64_t ld,
+                            float scale, float* out, int64_t ldo, hipStream_t s);
 hipError_t launch_row_softmax(int M, const int32_t* indptr, const float* in, float* out, float scale,
                               hipStream_t s);
 

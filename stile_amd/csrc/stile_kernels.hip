@@ -1046,6 +1046,179 @@ hipError_t launch_row_softmax(int M, const int32_t* indptr, const float* in, flo
 }
 
 // ---------------------------------------------------------------------------
+// Fused sparse attention: out = softmax(scale * Q K^T on the mask) V for BF16
+// Q, K, V (rows x 64, leading dimension ld), FP32 out, over a plan built by
+// stile_plan_create_attention (block-diagonal masks; tc_mask marks the mask
+// entries among each chunk's k-slots).  One wave per window of 16 query rows;
+// lane l holds query q = l & 15 and lane group g = l >> 4.
+//
+// Per chunk of 32 key columns (k-slots):
+//   S^T = K Q^T as two 16x16 tiles (keys x queries), two v_mfma_f32_16x16x32_
+//     bf16 each over the 64 features (lane group g carries features
+//     32s + 8g .. +7 of step s for both operands, one 16-byte load each);
+//     lane (g, q) then holds S[q][slot 16T + 4g + i], i = 0..3, T = 0, 1;
+//   online softmax per query (running max m and sum l; the chunk max is
+//     reduced over the four lanes of a query with xor shuffles);
+//   O^T += V^T P^T as four d-tiles of one v_mfma_f32_16x16x32_bf16: the
+//     lane's eight probabilities are its B operand as they are (k-slot e of
+//     lane group g = slot 4g + e for e < 4, 16 + 4g + e - 4 otherwise); the
+//     V rows of the chunk go through LDS transposed (Vt[d][slot]) so the A
+//     operand is two ds_read_b64 per d-tile.
+// Sparse entries of a window row (the plan's sparse part) follow one at a
+// time: the four lanes of the query share the dot product and the update.
+// Out row r = O[r] / l[r] (0 for a row without entries).
+// ---------------------------------------------------------------------------
+constexpr int kAttD = 64;
+
+__device__ __forceinline__ uint32_t bf16_bits_pair(float lo, float hi) { return bf16_pair(lo, hi); }
+
+__global__ __launch_bounds__(256) void attention_fused_kernel(SpmmPlanDev p, const __bf16* __restrict__ Qm,
+                                                              const __bf16* __restrict__ Km,
+                                                              const __bf16* __restrict__ Vm, int64_t ld,
+                                                              float scale, float* __restrict__ out, int64_t ldo) {
+  __shared__ uint2 vt_all[4][kAttD * 8];  // per wave: Vt[d][quad of slots], 4 BF16 each
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = blockIdx.x * 4 + wv;
+  if (w >= p.W) return;
+  uint2* vt = vt_all[wv];
+  const int q = lane & 15, g = lane >> 4;
+  const int row = p.win_rows[w * 16 + q];
+  const int64_t qrow = row >= 0 ? row : 0;  // padding query rows compute on row 0, store nothing
+  const bf16x8 qb0 = *reinterpret_cast<const bf16x8*>(Qm + qrow * ld + 8 * g);
+  const bf16x8 qb1 = *reinterpret_cast<const bf16x8*>(Qm + qrow * ld + 32 + 8 * g);
+  f32x4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float m = -INFINITY, lsum = 0.f;
+  const int c0 = p.tc_ptr[w], c1 = p.tc_ptr[w + 1];
+  for (int c = c0; c < c1; ++c) {
+    const int32_t* cc = p.tc_cols + (int64_t)c * 32;
+    // K rows of slots q and 16 + q (this lane's A-operand rows)
+    const int kc0 = cc[(q & 3) * 8 + (q >> 2)];
+    const int kc1 = cc[(q & 3) * 8 + 4 + (q >> 2)];
+    const bf16x8 k00 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc0 * ld + 8 * g);
+    const bf16x8 k01 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc0 * ld + 32 + 8 * g);
+    const bf16x8 k10 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc1 * ld + 8 * g);
+    const bf16x8 k11 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc1 * ld + 32 + 8 * g);
+    // V rows of slots 4a + u, features 8o .. 8o + 7 (a = lane >> 3, o = lane & 7)
+    const int a = lane >> 3, o = lane & 7;
+    uint4 vv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      vv[u] = *reinterpret_cast<const uint4*>(Vm + (int64_t)cc[u * 8 + a] * ld + 8 * o);
+    const uint32_t mw = p.tc_mask[(int64_t)c * 16 + q];
+    f32x4 st0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(k00, qb0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    st0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(k01, qb1, st0, 0, 0, 0);
+    f32x4 st1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(k10, qb0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    st1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(k11, qb1, st1, 0, 0, 0);
+    // Vt[d][slot] for d = 8o + e, slots 4a .. 4a + 3
+    {
+      const uint32_t* v0 = reinterpret_cast<const uint32_t*>(&vv[0]);
+      const uint32_t* v1 = reinterpret_cast<const uint32_t*>(&vv[1]);
+      const uint32_t* v2 = reinterpret_cast<const uint32_t*>(&vv[2]);
+      const uint32_t* v3 = reinterpret_cast<const uint32_t*>(&vv[3]);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int h = e & 1, dw = e >> 1;
+        vt[(8 * o + e) * 8 + a] = make_uint2(half_sel(v0[dw], v1[dw], h), half_sel(v2[dw], v3[dw], h));
+      }
+    }
+    // online softmax
+    float sv[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      sv[i] = (mw >> (4 * g + i)) & 1u ? st0[i] : -INFINITY;
+      sv[4 + i] = (mw >> (16 + 4 * g + i)) & 1u ? st1[i] : -INFINITY;
+    }
+    float cm = sv[0];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) cm = fmaxf(cm, sv[i]);
+    cm = fmaxf(cm, __shfl_xor(cm, 16));
+    cm = fmaxf(cm, __shfl_xor(cm, 32));
+    const float m_new = fmaxf(m, cm);
+    const float alpha = m_new == -INFINITY ? 1.f : __expf((m - m_new) * scale);
+    uint32_t pk[4];
+    float psum = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float pa = sv[2 * i] == -INFINITY ? 0.f : __expf((sv[2 * i] - m_new) * scale);
+      const float pb = sv[2 * i + 1] == -INFINITY ? 0.f : __expf((sv[2 * i + 1] - m_new) * scale);
+      pk[i] = bf16_bits_pair(pa, pb);
+      // the sum uses the BF16-rounded probabilities the MFMA multiplies
+      psum += __builtin_bit_cast(float, pk[i] << 16) + __builtin_bit_cast(float, pk[i] & 0xffff0000u);
+    }
+    lsum = lsum * alpha + psum;
+    m = m_new;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] *= alpha;
+    // LDS writes of this wave complete before its reads below
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+    const bf16x8 pb = __builtin_bit_cast(bf16x8, make_uint4(pk[0], pk[1], pk[2], pk[3]));
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int d = 16 * t + q;
+      const uint2 lo = vt[d * 8 + g], hi = vt[d * 8 + 4 + g];
+      const bf16x8 va = __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, pb, acc[t], 0, 0, 0);
+    }
+    // reads of Vt done before the next chunk overwrites it
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // sparse entries of the window rows, one per step for all 16 queries
+  {
+    const int r0 = p.res_ptr[w * 16 + q], r1 = p.res_ptr[w * 16 + q + 1];
+    int n = r1 - r0;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) n = max(n, __shfl_xor(n, off));
+    for (int j = 0; j < n; ++j) {
+      const bool on = r0 + j < r1;
+      const int col = on ? p.res_cols[r0 + j] : 0;
+      const bf16x8 ka = *reinterpret_cast<const bf16x8*>(Km + (int64_t)col * ld + 8 * g);
+      const bf16x8 kb = *reinterpret_cast<const bf16x8*>(Km + (int64_t)col * ld + 32 + 8 * g);
+      float dot = 0.f;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) dot += (float)qb0[e] * (float)ka[e] + (float)qb1[e] * (float)kb[e];
+      dot += __shfl_xor(dot, 16);
+      dot += __shfl_xor(dot, 32);
+      const float sval = on ? dot : -INFINITY;
+      const float m_new = fmaxf(m, sval);
+      const float alpha = m_new == -INFINITY ? 1.f : __expf((m - m_new) * scale);
+      const float pr = sval == -INFINITY ? 0.f : __expf((sval - m_new) * scale);
+      const float prb = __builtin_bit_cast(float, bf16_bits_pair(pr, 0.f) << 16);
+      lsum = lsum * alpha + (g == 0 ? prb : 0.f);
+      m = m_new;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint2 vr = *reinterpret_cast<const uint2*>(Vm + (int64_t)col * ld + 16 * t + 4 * g);
+        acc[t][0] = acc[t][0] * alpha + prb * __builtin_bit_cast(float, vr.x << 16);
+        acc[t][1] = acc[t][1] * alpha + prb * __builtin_bit_cast(float, vr.x & 0xffff0000u);
+        acc[t][2] = acc[t][2] * alpha + prb * __builtin_bit_cast(float, vr.y << 16);
+        acc[t][3] = acc[t][3] * alpha + prb * __builtin_bit_cast(float, vr.y & 0xffff0000u);
+      }
+    }
+  }
+  float ltot = lsum + __shfl_xor(lsum, 16);
+  ltot += __shfl_xor(ltot, 32);
+  if (row < 0) return;
+  const float inv = ltot > 0.f ? 1.f / ltot : 0.f;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    st4(out + (int64_t)row * ldo + 16 * t + 4 * g,
+        make_float4(acc[t][0] * inv, acc[t][1] * inv, acc[t][2] * inv, acc[t][3] * inv));
+}
+
+hipError_t launch_attention(const SpmmPlanDev& p, const void* Q, const void* K, const void* V, int64_t ld,
+                            float scale, float* out, int64_t ldo, hipStream_t s) {
+  if (p.W == 0) return hipSuccess;
+  hipLaunchKernelGGL(attention_fused_kernel, dim3((unsigned)((p.W + 3) / 4)), dim3(256), 0, s, p,
+                     static_cast<const __bf16*>(Q), static_cast<const __bf16*>(K), static_cast<const __bf16*>(V),
+                     ld, scale, out, ldo);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Value refresh.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void set_values_kernel(const float* __restrict__ vals,

@@ -128,6 +128,7 @@ struct SpmmBuild {
   std::vector<int32_t> s_id;
   int64_t block_nnz = 0;
   bool has_ids = true;     // every group carried a_ids
+  bool want_mask = false;  // also build tc_mask (fused attention)
   int64_t max_id = -1;     // largest value id (values array must be longer)
 };
 
@@ -427,9 +428,20 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   std::vector<unsigned char> tc_vals((size_t)nchunks * 64 * 8 * val_bytes);
   std::vector<float> tc_scale(blk_dtype == STILE_BLK_F16 ? nchunks : 0);
   std::vector<int32_t> tc_ids(bd.has_ids ? (size_t)nchunks * 64 * 8 : 0);
+  // tc_mask[c][r] bit s: k-slot s of chunk c holds an entry of window row r
+  std::vector<uint32_t> tc_mask(bd.want_mask ? (size_t)nchunks * 16 : 0);
   for (int32_t w = 0; w < W; ++w) {
     const auto& cols = bd.win_cols[w];
     for (int64_t c = tc_ptr[w]; c < tc_ptr[w + 1]; ++c) {
+      if (bd.want_mask) {
+        const size_t b0 = (size_t)(c - tc_ptr[w]) * 32;
+        for (int r = 0; r < 16; ++r) {
+          uint32_t bits = 0;
+          for (size_t ks = 0; ks < 32 && b0 + ks < cols.size(); ++ks)
+            if (cols[b0 + ks].id[r] >= 0 || cols[b0 + ks].v[r] != 0.f) bits |= 1u << ks;
+          tc_mask[(size_t)c * 16 + r] = bits;
+        }
+      }
       const size_t base = (size_t)(c - tc_ptr[w]) * 32;
       float scale = 1.f;
       if (blk_dtype == STILE_BLK_F16) {
@@ -483,6 +495,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const size_t o_tcids = lay.add(tc_ids.size() * 4);
   const size_t o_resids = lay.add(res_ids.size() * 4);
   const size_t o_mix = lay.add(mix_slot.size() * 4);
+  const size_t o_mask = lay.add(tc_mask.size() * 4);
   HostBlob blob;
   blob.data.assign(lay.bytes, 0);
   blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
@@ -499,6 +512,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   blob.put(o_tcids, tc_ids.data(), tc_ids.size() * 4);
   blob.put(o_resids, res_ids.data(), res_ids.size() * 4);
   blob.put(o_mix, mix_slot.data(), mix_slot.size() * 4);
+  blob.put(o_mask, tc_mask.data(), tc_mask.size() * 4);
 
   StilePlan* plan = new (std::nothrow) StilePlan();
   if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
@@ -518,6 +532,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.W_big = W_big;
   d.n_mix = (int32_t)mix_slot.size();
   d.mix_slot = dptr<int32_t>(plan, o_mix);
+  d.tc_mask = tc_mask.empty() ? nullptr : dptr<uint32_t>(plan, o_mask);
   d.nchunks = (int32_t)nchunks;
   d.npieces = (int32_t)piece_row.size();
   d.piece_row = dptr<int32_t>(plan, o_prow);
@@ -555,13 +570,15 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   return STILE_OK;
 }
 
-int create_spmm(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype, StilePlan** out) {
+int create_spmm(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype, StilePlan** out,
+                bool want_mask = false) {
   if (!out) return fail(STILE_E_INVALID, "null output pointer");
   *out = nullptr;
   if (!packs || count <= 0) return fail(STILE_E_INVALID, "no packs");
   if (blk_dtype < STILE_BLK_F32 || blk_dtype > STILE_BLK_F16)
     return fail(STILE_E_INVALID, "unknown blk_dtype");
   SpmmBuild bd;
+  bd.want_mask = want_mask;
   int64_t roff = 0, coff = 0, ioff = 0;
   for (int32_t i = 0; i < count; ++i) {
     const size_t w0 = bd.win_rows.size() / 16;
@@ -733,7 +750,7 @@ int create_sddmm(const StileSddmmPack* packs, int32_t count, int32_t blk_dtype, 
 
 extern "C" {
 
-int32_t stile_abi_version(void) { return 2; }
+int32_t stile_abi_version(void) { return 3; }
 
 const char* stile_last_error(void) { return g_err.c_str(); }
 
@@ -784,6 +801,29 @@ int stile_plan_create_spmm(const StileSpmmPack* pack, int32_t blk_dtype, StilePl
 int stile_plan_create_spmm_batched(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype,
                                    StilePlan** out) {
   return create_spmm(packs, count, blk_dtype, out);
+}
+
+int stile_plan_create_attention(const StileSpmmPack* packs, int32_t count, StilePlan** out) {
+  for (int32_t i = 0; packs && i < count; ++i)
+    if (packs[i].M != packs[i].K) return fail(STILE_E_INVALID, "attention masks must be square");
+  return create_spmm(packs, count, STILE_BLK_BF16, out, true);
+}
+
+int stile_attention_run(const StilePlan* plan, const void* Q, const void* K, const void* V, int64_t ld, int32_t d,
+                        float scale, float* out, int64_t ldo, void* stream) {
+  if (!plan || plan->kind != 1 || (plan->spmm.nchunks > 0 && !plan->spmm.tc_mask))
+    return fail(STILE_E_INVALID, "not an attention plan (stile_plan_create_attention)");
+  if (d != 64) return fail(STILE_E_UNSUPPORTED, "fused attention supports head dimension 64");
+  if (!(scale > 0.f)) return fail(STILE_E_INVALID, "scale must be positive");
+  if (plan->spmm.W == 0) return STILE_OK;
+  if (!Q || !K || !V || !out) return fail(STILE_E_INVALID, "null pointer");
+  if (ld < d || ld % 8 != 0 || ldo < d || ldo % 4 != 0)
+    return fail(STILE_E_INVALID, "leading dimensions: ld >= d and a multiple of 8, ldo >= d and a multiple of 4");
+  if ((reinterpret_cast<uintptr_t>(Q) | reinterpret_cast<uintptr_t>(K) | reinterpret_cast<uintptr_t>(V)) & 15 ||
+      reinterpret_cast<uintptr_t>(out) & 15)
+    return fail(STILE_E_INVALID, "Q, K, V and out must be 16-byte aligned");
+  hipError_t e = stile::launch_attention(plan->spmm, Q, K, V, ld, scale, out, ldo, (hipStream_t)stream);
+  return e == hipSuccess ? STILE_OK : hip_fail(e, "attention launch");
 }
 
 int stile_plan_create_sddmm(const StileSddmmPack* pack, int32_t blk_dtype, StilePlan** out) {

@@ -90,7 +90,7 @@ def test_library_exports_every_header_symbol():
     for s in syms:
         assert hasattr(lib, s), f"{s} declared in include/stile.h but not exported"
         assert s in _native._SIGNATURES, f"{s} has no ctypes signature"
-    assert lib.stile_abi_version() == 2
+    assert lib.stile_abi_version() == 3
 
 
 def test_abi_rejects_bad_arguments_without_a_device():

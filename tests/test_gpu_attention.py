@@ -45,11 +45,12 @@ def _masks(count=6, seed=0):
     return [datasets.attention_mask(int(L), band=32, n_random=4, rng=rng) for L in lens]
 
 
-@pytest.mark.parametrize("blk,tol", [("float32", 1e-5), ("bfloat16", 1e-2)])
-def test_batched_attention_eager_and_graph(cuda, blk, tol):
+@pytest.mark.parametrize("blk,tol,fused", [("float32", 1e-5, False), ("bfloat16", 1e-2, False),
+                                           ("bfloat16", 1e-2, True)])
+def test_batched_attention_eager_and_graph(cuda, blk, tol, fused):
     masks = _masks()
     d = 64
-    att = BatchedSparseAttention(masks, d, blk_dtype=blk, device=cuda, workers=4)
+    att = BatchedSparseAttention(masks, d, blk_dtype=blk, device=cuda, workers=4, fused=fused)
     offs = att.row_off
     n = int(offs[-1])
     rng = np.random.default_rng(1)
@@ -58,7 +59,7 @@ def test_batched_attention_eager_and_graph(cuda, blk, tol):
         return [torch.from_numpy(rng.standard_normal((n, d)).astype(np.float32)).to(torch.bfloat16) for _ in range(3)]
 
     Q, K, V = inputs()
-    print(f"attention eager blk={blk} instances={len(masks)} nnz={att.nnz}", flush=True)
+    print(f"attention eager blk={blk} fused={fused} instances={len(masks)} nnz={att.nnz}", flush=True)
     ref = _reference(masks, *(t.float().numpy() for t in (Q, K, V)), offs)
     out = att.forward(Q.to(cuda), K.to(cuda), V.to(cuda)).cpu().numpy()
     assert _rel(out, ref) <= tol
@@ -90,3 +91,30 @@ def test_row_softmax_kernel(cuda):
         w = np.exp(0.5 * vals[a:e].astype(np.float64) - 0.5 * vals[a:e].max())
         w /= w.sum()
         assert np.max(np.abs(got[a:e] - w)) <= 1e-6
+
+
+def test_fused_attention_with_sparse_part_and_empty_rows(cuda):
+    """Masks whose search leaves a sparse part (random columns) and rows
+    without entries: the fused kernel's per-row path and its zero rows."""
+    import scipy.sparse
+    from stile_amd import api
+    rng = np.random.default_rng(11)
+    m1 = datasets.attention_mask(333, band=8, n_random=6, rng=rng).tolil()
+    m1[5, :] = 0
+    m1[200:216, :] = 0
+    m1 = m1.tocsr()
+    m1.eliminate_zeros()
+    m2 = scipy.sparse.random(150, 150, density=0.03, format="csr", random_state=3, dtype=np.float32)
+    m2.data[:] = 1.0
+    masks = [m1, m2]
+    att = BatchedSparseAttention(masks, 64, device=cuda, workers=1, fused=True,
+                                 opts=api.SearchOptions(only_ELL=True))
+    assert att.att_plan.info()["sparse_nnz"] > 0
+    n = int(att.row_off[-1])
+    Q, K, V = (torch.from_numpy(rng.standard_normal((n, 64)).astype(np.float32)).to(torch.bfloat16)
+               for _ in range(3))
+    print("fused attention sparse part + empty rows", flush=True)
+    out = att.forward(Q.to(cuda), K.to(cuda), V.to(cuda)).cpu().numpy()
+    ref = _reference(masks, *(t.float().numpy() for t in (Q, K, V)), att.row_off)
+    assert _rel(out, ref) <= 1e-2
+    assert np.all(out[5] == 0) and np.all(out[200:216] == 0)
