@@ -919,12 +919,15 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
 
 // ---------------------------------------------------------------------------
 // Hybrid fused SDDMM.  Waves [0, waves1d) run the 1D entries (segments of LPE
-// lanes, one entry each); the remaining waves each run one dense 16x16 block:
+// lanes, kSddmmU consecutive entries each); the remaining waves each run one
+// dense 16x16 block:
 // lane l = 16g + j loads float4 slices of X[rows[j]] and Y[cols[j]] at
 // features kb + 4g .. kb + 4g + 3 and feeds component e to MFMA e
 // (v_mfma_f32_16x16x4_f32, so k-slot g of MFMA e is feature kb + 4g + e).
 // The 16x16 result goes through LDS to the sampled scatter.
 // ---------------------------------------------------------------------------
+constexpr int kSddmmU = 4;
+
 template <int LPE>
 __global__ __launch_bounds__(256) void hyb_sddmm_kernel(SddmmPlanDev p,
                                                         const float* __restrict__ X,
@@ -938,14 +941,54 @@ __global__ __launch_bounds__(256) void hyb_sddmm_kernel(SddmmPlanDev p,
   const int wib = threadIdx.x >> 6;
   const int wave = blockIdx.x * 4 + wib;
   if (wave < waves1d) {
+    // a segment of LPE lanes computes kSddmmU consecutive entries, their
+    // index loads and row gathers all in flight together
     constexpr int EPW = 64 / LPE;
     const int sub = lane % LPE;
-    const int e = wave * EPW + lane / LPE;
-    if (e >= p.n1d) return;
-    const int row = p.e_row[e], col = p.e_col[e];
-    const float s = segment_dot<LPE>(X + (int64_t)row * ldx,
-                                     Y + (int64_t)col * ldy, Kf, sub, vec);
-    if (sub == 0) out[p.e_out[e]] = s;
+    const int e0 = (wave * EPW + lane / LPE) * kSddmmU;
+    if (e0 >= p.n1d) return;
+    int row[kSddmmU], col[kSddmmU], dst[kSddmmU];
+#pragma unroll
+    for (int u = 0; u < kSddmmU; ++u) {
+      const int e = min(e0 + u, p.n1d - 1);
+      row[u] = p.e_row[e];
+      col[u] = p.e_col[e];
+      dst[u] = p.e_out[e];
+    }
+    float s[kSddmmU];
+#pragma unroll
+    for (int u = 0; u < kSddmmU; ++u) s[u] = 0.f;
+    if (vec) {
+      for (int k = 4 * sub; k < Kf; k += 4 * LPE) {
+        float4 a[kSddmmU], b[kSddmmU];
+#pragma unroll
+        for (int u = 0; u < kSddmmU; ++u) {
+          // consecutive entries of one row (CSR order) share the X load
+          if (u > 0 && row[u] == row[u - 1])
+            a[u] = a[u - 1];
+          else
+            a[u] = ld4(X + (int64_t)row[u] * ldx + k);
+          b[u] = ld4(Y + (int64_t)col[u] * ldy + k);
+        }
+#pragma unroll
+        for (int u = 0; u < kSddmmU; ++u) s[u] += a[u].x * b[u].x + a[u].y * b[u].y + a[u].z * b[u].z + a[u].w * b[u].w;
+      }
+    } else {
+      for (int k = sub; k < Kf; k += LPE) {
+#pragma unroll
+        for (int u = 0; u < kSddmmU; ++u) s[u] += X[(int64_t)row[u] * ldx + k] * Y[(int64_t)col[u] * ldy + k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSddmmU; ++u) {
+#pragma unroll
+      for (int off = LPE / 2; off > 0; off >>= 1) s[u] += __shfl_xor(s[u], off, LPE);
+    }
+    if (sub == 0) {
+#pragma unroll
+      for (int u = 0; u < kSddmmU; ++u)
+        if (e0 + u < p.n1d) out[dst[u]] = s[u];
+    }
     return;
   }
   const int b = wave - waves1d;
@@ -992,7 +1035,7 @@ hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,
   int l = 1;
   while (l * 2 <= lpe) l *= 2;
   lpe = l;
-  const int epw = 64 / lpe;
+  const int epw = 64 / lpe * kSddmmU;
   const int waves1d = (p.n1d + epw - 1) / epw;
   const int64_t waves = (int64_t)waves1d + p.nblk;
   if (waves == 0) return hipSuccess;
