@@ -1112,6 +1112,9 @@ hipError_t launch_row_softmax(int M, const int32_t* indptr, const float* in, flo
 // Out row r = O[r] / l[r] (0 for a row without entries).
 // ---------------------------------------------------------------------------
 constexpr int kAttD = 64;
+// Vt row stride in uint2 (4 BF16): 9, not 8, so that the eight d rows one
+// ds_write_b64 touches start on different LDS banks
+constexpr int kVtStride = 9;
 
 __device__ __forceinline__ uint32_t bf16_bits_pair(float lo, float hi) { return bf16_pair(lo, hi); }
 
@@ -1119,7 +1122,7 @@ __global__ __launch_bounds__(256) void attention_fused_kernel(SpmmPlanDev p, con
                                                               const __bf16* __restrict__ Km,
                                                               const __bf16* __restrict__ Vm, int64_t ld,
                                                               float scale, float* __restrict__ out, int64_t ldo) {
-  __shared__ uint2 vt_all[4][kAttD * 8];  // per wave: Vt[d][quad of slots], 4 BF16 each
+  __shared__ uint2 vt_all[4][kAttD * kVtStride];  // per wave: Vt[d][quad of slots], 4 BF16 each
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int w = blockIdx.x * 4 + wv;
   if (w >= p.W) return;
@@ -1134,22 +1137,33 @@ __global__ __launch_bounds__(256) void attention_fused_kernel(SpmmPlanDev p, con
   for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
   float m = -INFINITY, lsum = 0.f;
   const int c0 = p.tc_ptr[w], c1 = p.tc_ptr[w + 1];
-  for (int c = c0; c < c1; ++c) {
+  const int a = lane >> 3, o = lane & 7;
+  // the column ids this lane needs from a chunk: K rows of slots q and
+  // 16 + q (its A-operand rows), V rows of slots 4a + u (a = lane >> 3)
+  int ids[6];
+  auto load_ids = [&](int c) {
     const int32_t* cc = p.tc_cols + (int64_t)c * 32;
-    // K rows of slots q and 16 + q (this lane's A-operand rows)
-    const int kc0 = cc[(q & 3) * 8 + (q >> 2)];
-    const int kc1 = cc[(q & 3) * 8 + 4 + (q >> 2)];
+    ids[0] = cc[(q & 3) * 8 + (q >> 2)];
+    ids[1] = cc[(q & 3) * 8 + 4 + (q >> 2)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ids[2 + u] = cc[u * 8 + a];
+  };
+  if (c0 < c1) load_ids(c0);
+  for (int c = c0; c < c1; ++c) {
+    const int kc0 = ids[0], kc1 = ids[1];
     const bf16x8 k00 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc0 * ld + 8 * g);
     const bf16x8 k01 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc0 * ld + 32 + 8 * g);
     const bf16x8 k10 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc1 * ld + 8 * g);
     const bf16x8 k11 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc1 * ld + 32 + 8 * g);
-    // V rows of slots 4a + u, features 8o .. 8o + 7 (a = lane >> 3, o = lane & 7)
-    const int a = lane >> 3, o = lane & 7;
+    // V rows of slots 4a + u, features 8o .. 8o + 7
     uint4 vv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      vv[u] = *reinterpret_cast<const uint4*>(Vm + (int64_t)cc[u * 8 + a] * ld + 8 * o);
+      vv[u] = *reinterpret_cast<const uint4*>(Vm + (int64_t)ids[2 + u] * ld + 8 * o);
     const uint32_t mw = p.tc_mask[(int64_t)c * 16 + q];
+    // the next chunk's column ids, under this chunk's work (clamped: the
+    // last chunk reloads its own)
+    load_ids(min(c + 1, c1 - 1));
     f32x4 st0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(k00, qb0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     st0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(k01, qb1, st0, 0, 0, 0);
     f32x4 st1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(k10, qb0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
@@ -1163,7 +1177,7 @@ __global__ __launch_bounds__(256) void attention_fused_kernel(SpmmPlanDev p, con
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int h = e & 1, dw = e >> 1;
-        vt[(8 * o + e) * 8 + a] = make_uint2(half_sel(v0[dw], v1[dw], h), half_sel(v2[dw], v3[dw], h));
+        vt[(8 * o + e) * kVtStride + a] = make_uint2(half_sel(v0[dw], v1[dw], h), half_sel(v2[dw], v3[dw], h));
       }
     }
     // online softmax
@@ -1201,7 +1215,7 @@ __global__ __launch_bounds__(256) void attention_fused_kernel(SpmmPlanDev p, con
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int d = 16 * t + q;
-      const uint2 lo = vt[d * 8 + g], hi = vt[d * 8 + 4 + g];
+      const uint2 lo = vt[d * kVtStride + g], hi = vt[d * kVtStride + 4 + g];
       const bf16x8 va = __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
       acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, pb, acc[t], 0, 0, 0);
     }
