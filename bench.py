@@ -137,7 +137,10 @@ def _csr_dev(A, dev):
             torch.from_numpy(A.data.astype(np.float32)).to(dev))
 
 
-def _spmm_comparators(A, Bd, steps, warmup):
+def _spmm_comparators(A, Bd, steps, warmup, hyb=False):
+    """ms per launch of the plain CSR HIP kernel, the vendor CSR SpMM and,
+    with hyb, the SparseTIR-style ELL-bucket comparator (one column part,
+    buckets 1 .. 2^ceil(log2(nnz/rows)))."""
     import torch
     from stile_amd import _native
     ip, ix, vv = _csr_dev(A, Bd.device)
@@ -150,7 +153,12 @@ def _spmm_comparators(A, Bd, steps, warmup):
             ms_vendor = time_launches(vfn, steps, warmup)
         except Exception:                                        # noqa: BLE001
             ms_vendor = None
-    return ms_csr, ms_vendor
+    if not hyb:
+        return ms_csr, ms_vendor
+    from stile_amd.hyb import HybSpMM
+    h = HybSpMM(A, 1, device=Bd.device)
+    ms_hyb = time_launches(lambda: h.run(Bd, C2), steps, warmup)
+    return ms_csr, ms_vendor, ms_hyb
 
 
 def cache_path(args, name):
@@ -191,7 +199,7 @@ def run_pruned_layer(args, dev):
     C = torch.empty((M, N), device=dev, dtype=torch.float32)
     ms = time_launches(lambda: plan.run_spmm(Bd, C), args.steps, args.warmup)
     C_h = C.cpu().numpy()
-    ms_csr, ms_vendor = _spmm_comparators(A, Bd, max(args.steps // 4, 10), 5)
+    ms_csr, ms_vendor, ms_hyb = _spmm_comparators(A, Bd, max(args.steps // 4, 10), 5, hyb=True)
     flops = 2 * A.nnz * N
     n128 = None
     if not args.no_n128:
@@ -216,8 +224,10 @@ def run_pruned_layer(args, dev):
                 "global_batch": 1, "parallelism": "single GPU"},
                roofline=roofline(spmm_alg_bytes(A.nnz, M, K, N), ms, traffic),
                plain_csr_gflops=_gf(flops, ms_csr), vendor_csr_gflops=_gf(flops, ms_vendor),
+               sparsetir_hyb_gflops=_gf(flops, ms_hyb),
                hybrid_over_csr=round(ms_csr / ms, 3),
                hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),
+               hybrid_over_sparsetir_hyb=round(ms_hyb / ms, 3),
                n128=n128,
                hbm_copy_gbs={"value": round(copy_gbs, 1), "what": "256 MiB device copy (read + write)",
                              "achieved_over_copy": round(spmm_alg_bytes(A.nnz, M, K, N) / (ms * 1e-3) / 1e9
@@ -244,15 +254,17 @@ def run_uniform_spmm(args, dev):
     Bd = torch.from_numpy(B).to(dev)
     C = torch.empty((M, N), device=dev, dtype=torch.float32)
     ms = time_launches(lambda: plan.run_spmm(Bd, C), args.steps, args.warmup)
-    ms_csr, ms_vendor = _spmm_comparators(A, Bd, args.steps, args.warmup)
+    ms_csr, ms_vendor, ms_hyb = _spmm_comparators(A, Bd, args.steps, args.warmup, hyb=True)
     flops = 2 * A.nnz * N
     out = line("SpMM GFLOP/s (2*nnz*N), uniform 1% A", flops / (ms * 1e-3) / 1e9, "GFLOP/s", ms, args, "f32",
                {"workload": f"SpMM FP32, uniform A 4096x4096 nnz={A.nnz} x B 4096x{N}", "M": M, "K": K, "N": N,
                 "nnz": int(A.nnz), "parallelism": "single GPU"},
                roofline=roofline(spmm_alg_bytes(A.nnz, M, K, N), ms),
                plain_csr_gflops=_gf(flops, ms_csr), vendor_csr_gflops=_gf(flops, ms_vendor),
+               sparsetir_hyb_gflops=_gf(flops, ms_hyb),
                hybrid_over_csr=round(ms_csr / ms, 3),
                hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),
+               hybrid_over_sparsetir_hyb=round(ms_hyb / ms, 3),
                plan=dict(plan.info(), search_tune_and_plan_s=round(t_plan, 3), tuning=tuning))
     if not args.no_cpu:
         out["max_rel_err_vs_fp64"] = ops.max_rel_err(C.cpu().numpy(), ops.spmm_fp64(A, B))
@@ -335,7 +347,7 @@ def run_gcn(args, dev):
     ms_t = time_launches(lambda: op.plan_t.run_spmm(dY, C), steps, 2)
     sd = torch.empty(A.nnz, device=dev)
     ms_sd = time_launches(lambda: op.sddmm_plan.run_sddmm(dY, HW, sd), steps, 2)
-    ms_csr, ms_vendor = _spmm_comparators(A, HW, steps, 2)
+    ms_csr, ms_vendor, ms_hyb = _spmm_comparators(A, HW, steps, 2, hyb=True)
     ip, ix, _ = _csr_dev(A, dev)
     ms_csr_sd = time_launches(lambda: _native.csr_sddmm_f32(ip, ix, dY, HW, sd), steps, 2)
     flops = 3 * 2 * A.nnz * F
@@ -348,6 +360,7 @@ def run_gcn(args, dev):
                kernels_ms={"spmm_fwd": round(ms_fwd, 4), "spmm_transposed": round(ms_t, 4),
                            "sddmm": round(ms_sd, 4), "plain_csr_spmm": round(ms_csr, 4),
                            "vendor_csr_spmm": None if ms_vendor is None else round(ms_vendor, 4),
+                           "sparsetir_hyb_spmm": round(ms_hyb, 4),
                            "plain_csr_sddmm": round(ms_csr_sd, 4)},
                plan={"spmm": op.plan.info(), "sddmm": op.sddmm_plan.info(), "symmetric_reuse": op.symmetric,
                      "search_tune_and_plan_s": round(t_plan, 3), "tuning": op.tuning})

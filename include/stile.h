@@ -178,6 +178,24 @@ int stile_csr_sddmm_f32(int32_t M, int32_t N, int32_t nnz, int32_t Kf,
                         const float* X, int64_t ldx, const float* Y,
                         int64_t ldy, float* out, void* stream);
 
+/* One bucket of the ELL-bucket ("hyb") format of SparseTIR's comparator
+ * (SPMM/SPMMbench_spmm_hyb.py:156-316, column_part_hyb): `rows` ELL rows of
+ * `width` slots; device arrays row[rows] (output row of each ELL row),
+ * cols[rows][width], vals[rows][width] (0 in padding slots). */
+typedef struct StileEllBucket {
+  int32_t rows;
+  int32_t width;
+  const int32_t* row;
+  const int32_t* cols;
+  const float* vals;
+} StileEllBucket;
+
+/* C (M x N) = sum over buckets of their ELL products (FP32; C is zero-filled,
+ * then each bucket adds with atomicAdd, one launch per bucket).  The
+ * comparator standing in for the reference's SparseTIR hyb baseline. */
+int stile_ell_buckets_spmm_f32(const StileEllBucket* buckets, int32_t count, int32_t M, int32_t N, const float* B,
+                               int64_t ldb, float* C, int64_t ldc, void* stream);
+
 /* Row softmax over a CSR pattern (FP32): for every row i,
  * out[p] = exp(scale*in[p] - m_i) / sum_q exp(scale*in[q] - m_i) over the
  * row's nonzeros p, q (m_i = max of scale*in over the row); empty rows write

@@ -66,6 +66,11 @@ class StilePlanInfo(ctypes.Structure):
 
 
 # Every symbol of include/stile.h with its ctypes signature.
+class StileEllBucket(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int32), ("width", ctypes.c_int32), ("row", ctypes.c_void_p),
+                ("cols", ctypes.c_void_p), ("vals", ctypes.c_void_p)]
+
+
 _SIGNATURES = {
     "stile_csr_spmm_f32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -91,6 +96,9 @@ _SIGNATURES = {
                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                        ctypes.c_void_p, ctypes.c_void_p]),
     "stile_plan_set_values": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+    "stile_ell_buckets_spmm_f32": (ctypes.c_int, [ctypes.POINTER(StileEllBucket), ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                                  ctypes.c_int64, ctypes.c_void_p]),
     "stile_plan_create_attention": (ctypes.c_int, [ctypes.POINTER(StileSpmmPack), ctypes.c_int32,
                                                    ctypes.POINTER(ctypes.c_void_p)]),
     "stile_attention_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -205,6 +213,18 @@ def csr_sddmm_f32(indptr, indices, X, Y, out=None):
                                       X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0),
                                       o.data_ptr(), _stream(X)))
     return o
+
+
+def ell_buckets_spmm_f32(buckets, M, B, C):
+    """C = sum of ELL-bucket products (buckets: list of device (row, cols,
+    vals, width) tuples as made by stile_amd.hyb.HybSpMM)."""
+    _dev_check(B, C)
+    arr = (StileEllBucket * max(len(buckets), 1))(*[
+        StileEllBucket(rows=r.shape[0], width=w, row=r.data_ptr(), cols=c.data_ptr(), vals=v.data_ptr())
+        for r, c, v, w in buckets])
+    _check(load().stile_ell_buckets_spmm_f32(arr, len(buckets), M, B.shape[1], B.data_ptr(), B.stride(0),
+                                             C.data_ptr(), C.stride(0), _stream(B)))
+    return C
 
 
 def csr_row_softmax_f32(indptr, vals, scale=1.0, out=None):

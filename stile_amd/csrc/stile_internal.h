@@ -102,6 +102,8 @@ hipError_t launch_set_values(const SpmmPlanDev& p, const float* vals, const int3
 
 hipError_t launch_attention(const SpmmPlanDev& p, const void* Q, const void* K, const void* V, int64_t ld,
                             float scale, float* out, int64_t ldo, hipStream_t s);
+hipError_t launch_ell_bucket_spmm(int rows, int width, const int32_t* orow, const int32_t* cols, const float* vals,
+                                  const float* B, int64_t ldb, int N, float* C, int64_t ldc, hipStream_t s);
 hipError_t launch_row_softmax(int M, const int32_t* indptr, const float* in, float* out, float scale,
                               hipStream_t s);
 

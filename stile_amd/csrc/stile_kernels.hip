@@ -1276,6 +1276,65 @@ hipError_t launch_attention(const SpmmPlanDev& p, const void* Q, const void* K, 
 }
 
 // ---------------------------------------------------------------------------
+// ELL-bucket SpMM (the SparseTIR hyb comparator): every ELL row of a bucket
+// holds `width` (column, value) slots; C[row] += sum over slots v * B[col].
+// One 16-lane group per ELL row and 64 columns; rows of C appear in several
+// buckets (column parts, split long rows), so results are added with
+// atomicAdd into a zero-filled C.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void ell_bucket_spmm_kernel(int rows, int width, const int32_t* __restrict__ orow,
+                                                              const int32_t* __restrict__ cols,
+                                                              const float* __restrict__ vals,
+                                                              const float* __restrict__ B, int64_t ldb, int N,
+                                                              float* __restrict__ C, int64_t ldc, int nC,
+                                                              bool vec) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t item = wave * 4 + (lane >> 4);
+  if (item >= (int64_t)rows * nC) return;
+  const int r = (int)(item / nC);
+  const int col = (int)(item - (int64_t)r * nC) * 64 + 4 * (lane & 15);
+  if (col >= N) return;
+  const int32_t* cr = cols + (int64_t)r * width;
+  const float* vr = vals + (int64_t)r * width;
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+  int k = 0;
+  for (; k + 4 <= width; k += 4) {
+    const int k0 = cr[k], k1 = cr[k + 1], k2 = cr[k + 2], k3 = cr[k + 3];
+    const float v0 = vr[k], v1 = vr[k + 1], v2 = vr[k + 2], v3 = vr[k + 3];
+    const float4 b0 = load_b4(B, ldb, k0, col, N, vec), b1 = load_b4(B, ldb, k1, col, N, vec);
+    const float4 b2 = load_b4(B, ldb, k2, col, N, vec), b3 = load_b4(B, ldb, k3, col, N, vec);
+    s.x += v0 * b0.x + v1 * b1.x + v2 * b2.x + v3 * b3.x;
+    s.y += v0 * b0.y + v1 * b1.y + v2 * b2.y + v3 * b3.y;
+    s.z += v0 * b0.z + v1 * b1.z + v2 * b2.z + v3 * b3.z;
+    s.w += v0 * b0.w + v1 * b1.w + v2 * b2.w + v3 * b3.w;
+  }
+  for (; k < width; ++k) {
+    const float v = vr[k];
+    const float4 b = load_b4(B, ldb, cr[k], col, N, vec);
+    s.x += v * b.x; s.y += v * b.y; s.z += v * b.z; s.w += v * b.w;
+  }
+  float* c = C + (int64_t)orow[r] * ldc + col;
+  atomicAdd(c, s.x);
+  if (col + 1 < N) atomicAdd(c + 1, s.y);
+  if (col + 2 < N) atomicAdd(c + 2, s.z);
+  if (col + 3 < N) atomicAdd(c + 3, s.w);
+}
+
+hipError_t launch_ell_bucket_spmm(int rows, int width, const int32_t* orow, const int32_t* cols, const float* vals,
+                                  const float* B, int64_t ldb, int N, float* C, int64_t ldc, hipStream_t s) {
+  if (rows == 0 || N == 0) return hipSuccess;
+  const bool vec = (N % 4 == 0) && (ldb % 4 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
+  const int nC = (N + 63) / 64;
+  const int64_t items = (int64_t)rows * nC;
+  const int64_t blocks = (items + 15) / 16;
+  if (blocks > INT32_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(ell_bucket_spmm_kernel, dim3((unsigned)blocks), dim3(256), 0, s, rows, width, orow, cols, vals,
+                     B, ldb, N, C, ldc, nC, vec);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Value refresh.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void set_values_kernel(const float* __restrict__ vals,

@@ -794,6 +794,28 @@ int stile_csr_row_softmax_f32(int32_t M, const int32_t* indptr, const float* in,
   return e == hipSuccess ? STILE_OK : hip_fail(e, "row_softmax launch");
 }
 
+int stile_ell_buckets_spmm_f32(const StileEllBucket* buckets, int32_t count, int32_t M, int32_t N, const float* B,
+                               int64_t ldb, float* C, int64_t ldc, void* stream) {
+  if (M < 0 || N < 0 || count < 0) return fail(STILE_E_INVALID, "negative size");
+  if (M == 0 || N == 0) return STILE_OK;
+  if (!C || (count > 0 && (!buckets || !B))) return fail(STILE_E_INVALID, "null pointer");
+  if (ldb < N || ldc < N) return fail(STILE_E_INVALID, "leading dimension smaller than N");
+  for (int32_t i = 0; i < count; ++i) {
+    const StileEllBucket& b = buckets[i];
+    if (b.rows < 0 || b.width <= 0) return fail(STILE_E_INVALID, "bucket: rows < 0 or width <= 0");
+    if (b.rows > 0 && (!b.row || !b.cols || !b.vals)) return fail(STILE_E_INVALID, "bucket: null buffer");
+  }
+  hipError_t e = stile::launch_fill_zero_2d(C, M, N, ldc, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "fill_zero launch");
+  for (int32_t i = 0; i < count; ++i) {
+    const StileEllBucket& b = buckets[i];
+    e = stile::launch_ell_bucket_spmm(b.rows, b.width, b.row, b.cols, b.vals, B, ldb, N, C, ldc,
+                                      (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "ell_bucket_spmm launch");
+  }
+  return STILE_OK;
+}
+
 int stile_plan_create_spmm(const StileSpmmPack* pack, int32_t blk_dtype, StilePlan** out) {
   return create_spmm(pack, 1, blk_dtype, out);
 }

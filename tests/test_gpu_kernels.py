@@ -179,3 +179,16 @@ def test_hybrid_sddmm(cuda, Kf):
     ref = ops.sddmm_fp64(A, X, Y)
     assert ops.bound_ratio(out, ref, ops.sddmm_abs_bound(A, X, Y), Kf) <= 1.0
     assert ops.max_rel_err(out, ops.sddmm_ref_fp32(A, X, Y)) <= 1e-5
+
+
+@pytest.mark.parametrize("parts,N", [(1, 64), (4, 100)])
+def test_hyb_ell_bucket_comparator(cuda, parts, N):
+    from stile_amd import hyb
+    A = datasets.rmat_graph(11, seed=5)
+    A.data = np.random.default_rng(1).uniform(-1, 1, A.nnz).astype(np.float32)
+    B = np.random.default_rng(2).standard_normal((A.shape[1], N), dtype=np.float32)
+    print(f"hyb ELL-bucket comparator parts={parts} N={N}", flush=True)
+    h = hyb.HybSpMM(A, parts, device=cuda)
+    C = h.run(_t(B, cuda)).cpu().numpy()
+    ref64 = ops.spmm_fp64(A, B)
+    assert ops.max_rel_err(C, ref64) <= 1e-5
