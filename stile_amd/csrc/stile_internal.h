@@ -21,6 +21,12 @@ constexpr int kSplitNnz = 256;
 // through LDS in wave order.
 constexpr int kBigChunks = 8;
 constexpr int kBigWaves = 8;
+// Windows with more than kHugeChunks chunks (hub rows of power-law graphs)
+// are cut into parts of at most kHugeChunks chunks, one workgroup each, that
+// add their partial rows into C with atomicAdd (those rows are zeroed first):
+// one workgroup per window would leave its last wave running long after the
+// rest of the grid.  Windows up to kHugeChunks keep the fixed-order sums.
+constexpr int kHugeChunks = 256;
 
 // Device view of a hybrid SpMM plan (window-major execution layout).
 //
@@ -39,6 +45,11 @@ struct SpmmPlanDev {
   int32_t M, K, W;
   int32_t W_tc;              // windows [0, W_tc) hold dense chunks; the rest are sparse-only
   int32_t W_big;             // windows [0, W_big) hold more than kBigChunks chunks
+  int32_t W_huge;            // windows [0, W_huge) hold more than kHugeChunks chunks
+  int32_t n_parts;           // parts of the huge windows
+  const int32_t* part_win;   // [n_parts] window of each part
+  const int32_t* part_c0;    // [n_parts] chunk range of each part
+  const int32_t* part_c1;
   int32_t nchunks;
   int32_t blk_dtype;
   int32_t npieces;           // split pieces of long sparse rows
@@ -104,6 +115,7 @@ hipError_t launch_attention(const SpmmPlanDev& p, const void* Q, const void* K, 
                             float scale, float* out, int64_t ldo, hipStream_t s);
 hipError_t launch_ell_bucket_spmm(int rows, int width, const int32_t* orow, const int32_t* cols, const float* vals,
                                   const float* B, int64_t ldb, int N, float* C, int64_t ldc, hipStream_t s);
+hipError_t launch_zero_window_rows(const SpmmPlanDev& p, int nwin, float* C, int64_t ldc, int N, hipStream_t s);
 hipError_t launch_row_softmax(int M, const int32_t* indptr, const float* in, float* out, float scale,
                               hipStream_t s);
 

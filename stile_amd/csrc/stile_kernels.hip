@@ -567,7 +567,9 @@ __device__ __forceinline__ void hyb_spmm_window(const SpmmPlanDev& p, const floa
   store_window(C, ldc, N, col, vec, sr, acc);
 }
 
-// Blocks [0, big_blocks) each run one (big window, 64 columns): wave v takes
+// Blocks [0, n_parts * nC) each run one (part of a huge window, 64 columns)
+// and add into C with atomicAdd; the next big_blocks each run one (big
+// window, 64 columns): wave v takes
 // the v-th contiguous share of the window's chunks, waves 1.. leave their
 // partial sums in LDS and wave 0 adds them in wave order, then adds the
 // sparse parts and stores the 16 rows.
@@ -581,9 +583,46 @@ __global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p,
   __shared__ f32x4 part[kBigWaves - 1][4][64];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  if ((int)blockIdx.x < big_blocks) {
-    const int w = blockIdx.x / nC;
-    const int col = (blockIdx.x - w * nC) * 64 + 4 * (lane & 15);
+  const int huge_blocks = p.n_parts * nC;
+  if ((int)blockIdx.x < huge_blocks) {
+    // one part of a huge window: the partial rows are added into C (zeroed
+    // before this kernel) with atomicAdd
+    const int pt = blockIdx.x / nC;
+    const int w = p.part_win[pt];
+    const int col = (blockIdx.x - pt * nC) * 64 + 4 * (lane & 15);
+    const int c0 = p.part_c0[pt], c1 = p.part_c1[pt];
+    const int share = (c1 - c0 + kBigWaves - 1) / kBigWaves;
+    const int a = min(c0 + wv * share, c1), b = min(a + share, c1);
+    SlotRows sr;
+    if (wv == 0) sr = load_rows(p, w * 16 + (lane >> 4) * 4);
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    window_dense<MODE>(p, B, ldb, N, col, vec, a, b, lane, acc);
+    if (wv > 0) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) part[wv - 1][t][lane] = acc[t];
+    }
+    __syncthreads();
+    if (wv == 0) {
+      for (int v = 0; v < kBigWaves - 1; ++v) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] += part[v][t][lane];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (sr.row[i] < 0) continue;
+        float* cp = C + (int64_t)sr.row[i] * ldc + col;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (col + t < N) atomicAdd(cp + t, acc[t][i]);
+      }
+    }
+    return;
+  }
+  if ((int)blockIdx.x < huge_blocks + big_blocks) {
+    const int w = p.W_huge + (int)(blockIdx.x - huge_blocks) / nC;
+    const int col = (int)(blockIdx.x - huge_blocks - (w - p.W_huge) * nC) * 64 + 4 * (lane & 15);
     const int c0 = p.tc_ptr[w], c1 = p.tc_ptr[w + 1];
     const int share = (c1 - c0 + kBigWaves - 1) / kBigWaves;
     const int a = min(c0 + wv * share, c1), b = min(a + share, c1);
@@ -607,7 +646,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p,
     }
     return;
   }
-  const int64_t wave = (int64_t)(blockIdx.x - big_blocks) * kBigWaves + wv;
+  const int64_t wave = (int64_t)(blockIdx.x - huge_blocks - big_blocks) * kBigWaves + wv;
   if (wave >= (int64_t)(p.W_tc - p.W_big) * nC) return;
   const int w = p.W_big + (int)(wave / nC);
   hyb_spmm_window<MODE>(p, B, ldb, N, C, ldc, w, (int)(wave - (int64_t)(w - p.W_big) * nC) * 64, vec);
@@ -880,8 +919,13 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   } else if (p.W_tc > 0) {
-    const int64_t big_blocks = (int64_t)p.W_big * nC;
-    const int64_t blocks = big_blocks + ((int64_t)(p.W_tc - p.W_big) * nC + kBigWaves - 1) / kBigWaves;
+    if (p.n_parts > 0) {
+      hipError_t e = launch_zero_window_rows(p, p.W_huge, C, ldc, N, s);
+      if (e != hipSuccess) return e;
+    }
+    const int64_t big_blocks = (int64_t)(p.W_big - p.W_huge) * nC;
+    const int64_t blocks = (int64_t)p.n_parts * nC + big_blocks +
+                           ((int64_t)(p.W_tc - p.W_big) * nC + kBigWaves - 1) / kBigWaves;
     if (blocks > INT32_MAX) return hipErrorInvalidValue;
     dim3 grid((unsigned)blocks), block(kBigWaves * 64);
     const int bb = (int)big_blocks;
@@ -1380,6 +1424,22 @@ __global__ __launch_bounds__(256) void fill_zero_2d_kernel(float* __restrict__ p
     const int64_t r = i / cols, c = i - r * cols;
     p[r * ld + c] = 0.f;
   }
+}
+
+// C rows of windows [0, nwin) set to zero (the huge windows' rows, before
+// their parts add into them).  One block per row slot.
+__global__ __launch_bounds__(256) void zero_window_rows_kernel(const int32_t* __restrict__ win_rows, float* C,
+                                                               int64_t ldc, int N) {
+  const int row = win_rows[blockIdx.x];
+  if (row < 0) return;
+  float* c = C + (int64_t)row * ldc;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) c[i] = 0.f;
+}
+
+hipError_t launch_zero_window_rows(const SpmmPlanDev& p, int nwin, float* C, int64_t ldc, int N, hipStream_t s) {
+  if (nwin == 0 || N == 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_window_rows_kernel, dim3((unsigned)(nwin * 16)), dim3(256), 0, s, p.win_rows, C, ldc, N);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_zero_2d(float* p, int rows, int cols, int64_t ld, hipStream_t s) {

@@ -345,6 +345,8 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
                                        [&](int32_t w) { return nch(w) > stile::kBigChunks; });
   std::stable_sort(order.begin(), big_end, [&](int32_t a, int32_t b) { return nch(a) > nch(b); });
   const int32_t W_big = (int32_t)(big_end - order.begin());
+  int32_t W_huge = 0;
+  while (W_huge < W_big && nch(order[W_huge]) > stile::kHugeChunks) ++W_huge;
   {
     std::vector<int32_t> new_of(W);
     for (int32_t nw = 0; nw < W; ++nw) new_of[order[nw]] = nw;
@@ -423,6 +425,14 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
     tc_ptr[w + 1] = (int32_t)nchunks;
   }
   if (nchunks > INT32_MAX / 64) return fail(STILE_E_UNSUPPORTED, "too many dense chunks");
+  // Parts of the huge windows: consecutive ranges of at most kHugeChunks chunks.
+  std::vector<int32_t> part_win, part_c0, part_c1;
+  for (int32_t w = 0; w < W_huge; ++w)
+    for (int64_t c = tc_ptr[w]; c < tc_ptr[w + 1]; c += stile::kHugeChunks) {
+      part_win.push_back(w);
+      part_c0.push_back((int32_t)c);
+      part_c1.push_back((int32_t)std::min<int64_t>(c + stile::kHugeChunks, tc_ptr[w + 1]));
+    }
   const size_t val_bytes = blk_dtype == STILE_BLK_F32 ? 4 : 2;
   std::vector<int32_t> tc_cols((size_t)nchunks * 32);
   std::vector<unsigned char> tc_vals((size_t)nchunks * 64 * 8 * val_bytes);
@@ -496,6 +506,9 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const size_t o_resids = lay.add(res_ids.size() * 4);
   const size_t o_mix = lay.add(mix_slot.size() * 4);
   const size_t o_mask = lay.add(tc_mask.size() * 4);
+  const size_t o_pwin = lay.add(part_win.size() * 4);
+  const size_t o_pc0 = lay.add(part_c0.size() * 4);
+  const size_t o_pc1 = lay.add(part_c1.size() * 4);
   HostBlob blob;
   blob.data.assign(lay.bytes, 0);
   blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
@@ -513,6 +526,9 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   blob.put(o_resids, res_ids.data(), res_ids.size() * 4);
   blob.put(o_mix, mix_slot.data(), mix_slot.size() * 4);
   blob.put(o_mask, tc_mask.data(), tc_mask.size() * 4);
+  blob.put(o_pwin, part_win.data(), part_win.size() * 4);
+  blob.put(o_pc0, part_c0.data(), part_c0.size() * 4);
+  blob.put(o_pc1, part_c1.data(), part_c1.size() * 4);
 
   StilePlan* plan = new (std::nothrow) StilePlan();
   if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
@@ -530,6 +546,11 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.W = W;
   d.W_tc = W_tc;
   d.W_big = W_big;
+  d.W_huge = W_huge;
+  d.n_parts = (int32_t)part_win.size();
+  d.part_win = dptr<int32_t>(plan, o_pwin);
+  d.part_c0 = dptr<int32_t>(plan, o_pc0);
+  d.part_c1 = dptr<int32_t>(plan, o_pc1);
   d.n_mix = (int32_t)mix_slot.size();
   d.mix_slot = dptr<int32_t>(plan, o_mix);
   d.tc_mask = tc_mask.empty() ? nullptr : dptr<uint32_t>(plan, o_mask);

@@ -145,3 +145,21 @@ def test_tuned_plan_is_correct_and_reports_every_candidate(cuda):
     got = splan.run_sddmm(_t(X, cuda), _t(Y, cuda)).cpu().numpy()
     ref = ops.sddmm_fp64(A, X, Y)
     assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("blk", ["float32", "bfloat16"])
+def test_spmm_huge_windows_split_into_parts(cuda, blk):
+    """Windows with more than 256 dense chunks (hub rows) run as several
+    workgroups adding into zeroed rows with atomics."""
+    A = scipy.sparse.random(40, 20000, density=0.06, format="csr", random_state=8, dtype=np.float32)
+    B = np.random.default_rng(8).standard_normal((20000, 72), dtype=np.float32)
+    print(f"huge windows blk={blk}", flush=True)
+    plan = api.build_spmm_plan(A, blk_dtype=blk, opts=api.SearchOptions(only_TC=True), N=72)
+    info = plan.info()
+    assert info["num_chunks"] > 2 * 256
+    C = plan.run_spmm(_t(B, cuda)).cpu().numpy()
+    tol = 1e-5 if blk == "float32" else 1e-2
+    assert ops.max_rel_err(C, ops.spmm_fp64(A, B)) <= tol
+    # a second run: the zeroing before the atomics makes runs independent
+    C2 = plan.run_spmm(_t(B, cuda)).cpu().numpy()
+    assert ops.max_rel_err(C2, ops.spmm_fp64(A, B)) <= tol
