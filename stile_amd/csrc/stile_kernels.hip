@@ -852,14 +852,28 @@ __global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, cons
   const int col = n0 + 4 * (lane & 15);
   const int beg = p.piece_beg[pc], end = p.piece_end[pc];
   float4 s = make_float4(0.f, 0.f, 0.f, 0.f), cc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int q = beg + g; q < end; q += 4) {
-    const int k0 = p.res_cols[q];
-    const float v0 = p.res_vals[q];
-    const float4 b0 = load_b4(B, ldb, k0, col, N, vec);
-    neu_add(s.x, cc.x, v0 * b0.x);
-    neu_add(s.y, cc.y, v0 * b0.y);
-    neu_add(s.z, cc.z, v0 * b0.z);
-    neu_add(s.w, cc.w, v0 * b0.w);
+  // eight gathers in flight per lane group (positions past the end read the
+  // piece's last entry with weight 0)
+  for (int q0 = beg + g; q0 < end; q0 += 32) {
+    int k[8];
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int q = q0 + 4 * i;
+      const int qc = min(q, end - 1);
+      k[i] = p.res_cols[qc];
+      v[i] = q < end ? p.res_vals[qc] : 0.f;
+    }
+    float4 b[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b[i] = load_b4(B, ldb, k[i], col, N, vec);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      neu_add(s.x, cc.x, v[i] * b[i].x);
+      neu_add(s.y, cc.y, v[i] * b[i].y);
+      neu_add(s.z, cc.z, v[i] * b[i].z);
+      neu_add(s.w, cc.w, v[i] * b[i].w);
+    }
   }
   s.x += cc.x; s.y += cc.y; s.z += cc.z; s.w += cc.w;
 #pragma unroll
