@@ -211,6 +211,10 @@ def run_pruned_layer(args, dev):
                 "vendor_us": None if ms128_vendor is None else round(ms128_vendor * 1e3, 2),
                 "hybrid_gflops": _gf(2 * A.nnz * 128, ms128),
                 "hbm_frac": roofline(spmm_alg_bytes(A.nnz, M, K, 128), ms128)["frac"]}
+    # back-to-back launch floor (a one-element kernel): what a launch costs
+    # before any work; it bounds small-N launches
+    tiny = torch.zeros(1, device=dev)
+    ms_floor = time_launches(lambda: tiny.add_(1.0), 200, 20)
     # measured device-to-device copy bandwidth (the practical HBM ceiling)
     cp_src = torch.empty(64 << 20, device=dev, dtype=torch.float32)
     cp_dst = torch.empty_like(cp_src)
@@ -228,7 +232,7 @@ def run_pruned_layer(args, dev):
                hybrid_over_csr=round(ms_csr / ms, 3),
                hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),
                hybrid_over_sparsetir_hyb=round(ms_hyb / ms, 3),
-               n128=n128,
+               n128=n128, launch_floor_us=round(ms_floor * 1e3, 2),
                hbm_copy_gbs={"value": round(copy_gbs, 1), "what": "256 MiB device copy (read + write)",
                              "achieved_over_copy": round(spmm_alg_bytes(A.nnz, M, K, N) / (ms * 1e-3) / 1e9
                                                          / copy_gbs, 4)},
