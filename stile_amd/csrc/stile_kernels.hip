@@ -1206,22 +1206,28 @@ __global__ __launch_bounds__(256) void attention_fused_kernel(SpmmPlanDev p, con
 #pragma unroll
     for (int u = 0; u < 4; ++u) ids[2 + u] = cc[u * 8 + a];
   };
-  if (c0 < c1) load_ids(c0);
-  for (int c = c0; c < c1; ++c) {
-    const int kc0 = ids[0], kc1 = ids[1];
-    const bf16x8 k00 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc0 * ld + 8 * g);
-    const bf16x8 k01 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc0 * ld + 32 + 8 * g);
-    const bf16x8 k10 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc1 * ld + 8 * g);
-    const bf16x8 k11 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)kc1 * ld + 32 + 8 * g);
-    // V rows of slots 4a + u, features 8o .. 8o + 7
+  // K and V of chunk c + 1 are requested before chunk c is processed (two
+  // register sets, loop unrolled by two so neither is copied); column ids run
+  // one chunk further ahead.
+  struct AttRegs {
+    bf16x8 k00, k01, k10, k11;
     uint4 vv[4];
+    uint32_t mw;
+  };
+  auto load_regs = [&](int c, AttRegs& r) {
+    r.k00 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)ids[0] * ld + 8 * g);
+    r.k01 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)ids[0] * ld + 32 + 8 * g);
+    r.k10 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)ids[1] * ld + 8 * g);
+    r.k11 = *reinterpret_cast<const bf16x8*>(Km + (int64_t)ids[1] * ld + 32 + 8 * g);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      vv[u] = *reinterpret_cast<const uint4*>(Vm + (int64_t)ids[2 + u] * ld + 8 * o);
-    const uint32_t mw = p.tc_mask[(int64_t)c * 16 + q];
-    // the next chunk's column ids, under this chunk's work (clamped: the
-    // last chunk reloads its own)
-    load_ids(min(c + 1, c1 - 1));
+      r.vv[u] = *reinterpret_cast<const uint4*>(Vm + (int64_t)ids[2 + u] * ld + 8 * o);
+    r.mw = p.tc_mask[(int64_t)c * 16 + q];
+  };
+  auto process = [&](const AttRegs& r) {
+    const bf16x8 &k00 = r.k00, &k01 = r.k01, &k10 = r.k10, &k11 = r.k11;
+    const uint4* vv = r.vv;
+    const uint32_t mw = r.mw;
     f32x4 st0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(k00, qb0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     st0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(k01, qb1, st0, 0, 0, 0);
     f32x4 st1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(k10, qb0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
@@ -1280,6 +1286,25 @@ __global__ __launch_bounds__(256) void attention_fused_kernel(SpmmPlanDev p, con
     // reads of Vt done before the next chunk overwrites it
     __builtin_amdgcn_s_waitcnt(0xc07f);
     __builtin_amdgcn_wave_barrier();
+    };
+  if (c0 < c1) {
+    const int last = c1 - 1;
+    AttRegs R0, R1;
+    load_ids(c0);
+    load_regs(c0, R0);
+    load_ids(min(c0 + 1, last));
+    int c = c0;
+    while (c + 2 <= c1) {
+      load_regs(min(c + 1, last), R1);
+      load_ids(min(c + 2, last));
+      process(R0);
+      ++c;
+      load_regs(min(c + 1, last), R0);
+      load_ids(min(c + 2, last));
+      process(R1);
+      ++c;
+    }
+    if (c < c1) process(R0);
   }
   // sparse entries of the window rows, one per step for all 16 queries
   {
