@@ -116,6 +116,23 @@ def cpu_spmm_baseline(A, B, cols, seconds, label):
                                       f"{n} runs of {dt * 1e3:.1f} ms"}
 
 
+def cpu_torch_csr_baseline(A, B, cols, seconds):
+    """torch's CPU CSR x dense product on every host thread, reported beside
+    the one-core scipy baseline so the host side is not judged on one core."""
+    import warnings
+    warnings.filterwarnings("ignore", message="Sparse CSR tensor support is in beta")
+    import numpy as np
+    import torch
+    At = torch.sparse_csr_tensor(torch.from_numpy(A.indptr.astype(np.int64)),
+                                 torch.from_numpy(A.indices.astype(np.int64)),
+                                 torch.from_numpy(A.data.astype(np.float32)), size=A.shape)
+    Bs = torch.from_numpy(np.ascontiguousarray(B[:, :cols]))
+    dt, n = _repeat_for(lambda: torch.sparse.mm(At, Bs), seconds)
+    return {"value": round(2 * A.nnz * Bs.shape[1] / dt / 1e9, 3), "unit": "GFLOP/s",
+            "cores": torch.get_num_threads(), "kind": "torch CPU CSR",
+            "sample": f"torch.sparse.mm(CSR A, B[:, :{Bs.shape[1]}]) FP32, {n} runs of {dt * 1e3:.1f} ms"}
+
+
 def vendor_csr_spmm(A, Bd):
     """torch.sparse CSR x dense (rocSPARSE) on the same A; None if unavailable."""
     import numpy as np
@@ -242,6 +259,7 @@ def run_pruned_layer(args, dev):
     if not args.no_cpu:
         out["max_rel_err_vs_fp64"] = ops.max_rel_err(C_h[:, :256], ops.spmm_fp64(A, B[:, :256]))
         out["cpu_baseline"] = cpu_spmm_baseline(A, B, 512, 10.0, "3072x768")
+        out["cpu_torch_csr_all_threads"] = cpu_torch_csr_baseline(A, B, 512, 5.0)
     return out
 
 

@@ -1174,8 +1174,6 @@ constexpr int kAttD = 64;
 // ds_write_b64 touches start on different LDS banks
 constexpr int kVtStride = 9;
 
-__device__ __forceinline__ uint32_t bf16_bits_pair(float lo, float hi) { return bf16_pair(lo, hi); }
-
 __global__ __launch_bounds__(256) void attention_fused_kernel(SpmmPlanDev p, const __bf16* __restrict__ Qm,
                                                               const __bf16* __restrict__ Km,
                                                               const __bf16* __restrict__ Vm, int64_t ld,
@@ -1264,7 +1262,7 @@ __global__ __launch_bounds__(256) void attention_fused_kernel(SpmmPlanDev p, con
     for (int i = 0; i < 4; ++i) {
       const float pa = sv[2 * i] == -INFINITY ? 0.f : __expf((sv[2 * i] - m_new) * scale);
       const float pb = sv[2 * i + 1] == -INFINITY ? 0.f : __expf((sv[2 * i + 1] - m_new) * scale);
-      pk[i] = bf16_bits_pair(pa, pb);
+      pk[i] = bf16_pair(pa, pb);
       // the sum uses the BF16-rounded probabilities the MFMA multiplies
       psum += __builtin_bit_cast(float, pk[i] << 16) + __builtin_bit_cast(float, pk[i] & 0xffff0000u);
     }
@@ -1326,7 +1324,7 @@ __global__ __launch_bounds__(256) void attention_fused_kernel(SpmmPlanDev p, con
       const float m_new = fmaxf(m, sval);
       const float alpha = m_new == -INFINITY ? 1.f : __expf((m - m_new) * scale);
       const float pr = sval == -INFINITY ? 0.f : __expf((sval - m_new) * scale);
-      const float prb = __builtin_bit_cast(float, bf16_bits_pair(pr, 0.f) << 16);
+      const float prb = __builtin_bit_cast(float, bf16_pair(pr, 0.f) << 16);
       lsum = lsum * alpha + (g == 0 ? prb : 0.f);
       m = m_new;
 #pragma unroll
