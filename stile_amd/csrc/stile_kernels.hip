@@ -748,7 +748,11 @@ __device__ __forceinline__ ChunkRegs load_chunk_regs(const SpmmPlanDev& p, int c
   return r;
 }
 
-__global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p, const float* __restrict__ B,
+// tc_ptr is p.tc_ptr, passed again as a restrict parameter so that the
+// compiler may read it with scalar loads inside the loop that stores C.
+__global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p,
+                                                                        const int32_t* __restrict__ tc_ptr,
+                                                                        const float* __restrict__ B,
                                                                         int64_t ldb, int N, float* __restrict__ C,
                                                                         int64_t ldc, int nC, int wpg, bool vec) {
   extern __shared__ uint2 bsh[];  // [K][16]: 4 BF16 per entry
@@ -773,26 +777,30 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
     }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  // The wave index is made visibly uniform so that the window and chunk
+  // bookkeeping below lives in scalar registers and reads tc_ptr with scalar
+  // loads: a vector load of the next window bound inside the loop would make
+  // every step wait for all outstanding vector loads and stores.
+  const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = lane >> 4, j = lane & 15;
   const int col = n0 + 4 * j;
   // This wave's contiguous run of windows, balanced by chunk count.
   const int wg_beg = grp * wpg;
   const int wg_end = min(wg_beg + wpg, p.W_tc);
   if (wg_beg >= wg_end) return;
-  const int cb = p.tc_ptr[wg_beg], ce = p.tc_ptr[wg_end];
+  const int cb = tc_ptr[wg_beg], ce = tc_ptr[wg_end];
   auto window_at = [&](int target) {  // first window w in [wg_beg, wg_end] with tc_ptr[w] >= target
     int lo = wg_beg, hi = wg_end;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
-      if (p.tc_ptr[mid] < target) lo = mid + 1; else hi = mid;
+      if (tc_ptr[mid] < target) lo = mid + 1; else hi = mid;
     }
     return lo;
   };
   const int w_beg = window_at(cb + (int)(((int64_t)(ce - cb) * wib) / kWaves));
   const int w_end = window_at(cb + (int)(((int64_t)(ce - cb) * (wib + 1)) / kWaves));
   if (w_beg >= w_end) return;
-  const int c_beg = p.tc_ptr[w_beg], c_end = p.tc_ptr[w_end];
+  const int c_beg = tc_ptr[w_beg], c_end = tc_ptr[w_end];
 
   // The operands of chunk c + 2 are requested while chunk c is multiplied
   // (indices clamped to the wave's last chunk, so every load is
@@ -802,7 +810,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   // ends before each chunk.
   const int last = c_end - 1;
   int w = w_beg;
-  int w_next = p.tc_ptr[w + 1];
+  int w_next = tc_ptr[w + 1];
   SlotRows sr = load_rows(p, w * 16 + g * 4);
   f32x4 acc[4];
 #pragma unroll
@@ -815,7 +823,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
     fill = load_chunk_regs(p, min(c + 2, last), g, lane);
     if (c >= w_next) {  // window w is complete: store it
       const SlotRows sr_next = load_rows(p, (w + 1) * 16 + g * 4);
-      const int w_next2 = p.tc_ptr[w + 2];
+      const int w_next2 = tc_ptr[w + 2];
       store_window<true>(C, ldc, N, col, vec, sr, acc);
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -928,8 +936,8 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
     const int g = std::min(groups, p.W_tc);
     const int wpg = (p.W_tc + g - 1) / g;
     const int grid = nC * ((p.W_tc + wpg - 1) / wpg);
-    hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel, dim3(grid), dim3(kLdsThreads), (size_t)p.K * 128, s, p, B, ldb, N, C,
-                       ldc, nC, wpg, vec);
+    hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel, dim3(grid), dim3(kLdsThreads), (size_t)p.K * 128, s, p, p.tc_ptr, B, ldb, N,
+                       C, ldc, nC, wpg, vec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   } else if (p.W_tc > 0) {
