@@ -339,7 +339,7 @@ def run_gcn(args, dev):
     import numpy as np
     import torch
     from stile_amd import _native, datasets
-    from stile_amd.autograd import SparseOperand
+    from stile_amd.autograd import SparseOperand, gcn_layer
     A, H, W = datasets.gcn_graph(args.gcn_scale)
     M = A.shape[0]
     F = H.shape[1]
@@ -359,7 +359,7 @@ def run_gcn(args, dev):
         Hd.grad = None
         Wd.grad = None
         vals.grad = None
-        y = op.spmm(Hd @ Wd, vals)
+        y = gcn_layer(op, Hd, Wd, vals)
         y.backward(dY)
     steps = max(args.steps // 20, 5)
     ms = time_launches(step, steps, 2)

@@ -181,6 +181,46 @@ def sddmm(op, X, Y):
     return op.sddmm(X, Y)
 
 
+def _split_k_tn(H, G, slab_rows=16384):
+    """H^T G for tall H, G (M x F, M >> F).  As one GEMM the reduction over M
+    leaves the library a handful of F x F output tiles for the whole GPU; cut
+    into M / slab_rows slabs it is one batched product with that many times
+    more tiles, then a sum over the slabs.  FP32 only (other dtypes, and M not
+    a multiple of the slab, take the single GEMM)."""
+    M = H.shape[0]
+    if H.dtype != torch.float32 or G.dtype != torch.float32 or M % slab_rows or M < 4 * slab_rows:
+        return H.t() @ G
+    S = M // slab_rows
+    H3 = H.contiguous().view(S, slab_rows, H.shape[1])
+    G3 = G.contiguous().view(S, slab_rows, G.shape[1])
+    return torch.bmm(H3.transpose(1, 2), G3).sum(0)
+
+
+class _TallMatmul(torch.autograd.Function):
+    """Y = H W for a tall H; the weight gradient H^T dY runs split over row
+    slabs (see _split_k_tn), the input gradient dY W^T as one GEMM."""
+
+    @staticmethod
+    def forward(ctx, H, W):
+        ctx.save_for_backward(H, W)
+        return H @ W
+
+    @staticmethod
+    def backward(ctx, dY):
+        H, W = ctx.saved_tensors
+        dH = dW = None
+        if ctx.needs_input_grad[0]:
+            dH = dY @ W.t()
+        if ctx.needs_input_grad[1]:
+            dW = _split_k_tn(H, dY)
+        return dH, dW
+
+
+def tall_matmul(H, W):
+    return _TallMatmul.apply(H, W)
+
+
 def gcn_layer(op, H, W, values=None):
-    """One GCN layer A.(H.W) (the GCN workload's forward)."""
-    return op.spmm(H @ W, values)
+    """One GCN layer A.(H.W) (the GCN workload's forward); H.W is a
+    tall_matmul, so the layer's weight gradient is reduced in row slabs."""
+    return op.spmm(tall_matmul(H, W), values)
