@@ -104,6 +104,20 @@ def _repeat_for(fn, seconds, max_runs=200):
     return (time.perf_counter() - t0) / n, n
 
 
+def host_cpu():
+    """The host the CPU baselines ran on: logical CPUs and model string."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
 def cpu_spmm_baseline(A, B, cols, seconds, label):
     """The reference's CPU path (scipy csr.astype(float32).dot(B), one core)
     on a bounded sample: the first `cols` columns of B."""
@@ -113,7 +127,7 @@ def cpu_spmm_baseline(A, B, cols, seconds, label):
     dt, n = _repeat_for(lambda: ops.spmm_ref_fp32(A, Bs), seconds)
     return {"value": round(2 * A.nnz * Bs.shape[1] / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
             "kind": "port", "sample": f"scipy CSR({label}, nnz={A.nnz}) . B[:, :{Bs.shape[1]}] FP32, "
-                                      f"{n} runs of {dt * 1e3:.1f} ms"}
+                                      f"{n} runs of {dt * 1e3:.1f} ms", "host": host_cpu()}
 
 
 def cpu_torch_csr_baseline(A, B, cols, seconds):
@@ -331,7 +345,15 @@ def run_rmat_sddmm(args, dev):
         out["cpu_baseline"] = {"value": round(2 * s * Kf / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
                                "kind": "port",
                                "sample": f"numpy (X[row]*Y[col]).sum(1) over the first {s} nonzeros, "
-                                         f"{n} runs of {dt * 1e3:.1f} ms"}
+                                         f"{n} runs of {dt * 1e3:.1f} ms", "host": host_cpu()}
+        Xt, Yt = torch.from_numpy(X), torch.from_numpy(Y)
+        rt = torch.from_numpy(rows.astype(np.int64))
+        ct = torch.from_numpy(A.indices.astype(np.int64))
+        dt2, n2 = _repeat_for(lambda: (Xt[rt] * Yt[ct]).sum(1), 5.0)
+        out["cpu_torch_all_threads"] = {"value": round(2 * A.nnz * Kf / dt2 / 1e9, 3), "unit": "GFLOP/s",
+                                        "cores": torch.get_num_threads(), "kind": "torch CPU",
+                                        "sample": f"torch (X[row]*Y[col]).sum(1) over all {A.nnz} nonzeros, "
+                                                  f"{n2} runs of {dt2 * 1e3:.1f} ms"}
     return out
 
 
