@@ -64,6 +64,7 @@ struct SpmmPlanDev {
   const int32_t* res_ptr;    // [W*16+1]
   const int32_t* res_cols;
   const float* res_vals;
+  int32_t nnz_sparse_only;   // sparse nonzeros in windows [W_tc, W)
   int32_t n_mix;             // slots of dense windows with a sparse part
   const int32_t* mix_slot;   // [n_mix]
   const uint32_t* tc_mask;   // [nchunks][16] entry bits (attention plans), else nullptr

@@ -657,9 +657,11 @@ __global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p,
 // then the sparse-only windows [W_tc, W) (four waves per window and 64
 // columns).  A kernel of its own so that its small register footprint sets
 // its occupancy (gathers in flight per CU), not the dense path's.
+// so_waves: waves for the sparse-only windows (0 when hyb_spmm_sparse_split_kernel
+// stores those rows instead).
 __global__ __launch_bounds__(256) void hyb_spmm_sparse_kernel(SpmmPlanDev p, const float* __restrict__ B,
                                                               int64_t ldb, int N, float* __restrict__ C,
-                                                              int64_t ldc, int nC, bool vec) {
+                                                              int64_t ldc, int nC, bool vec, int64_t so_waves) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int64_t mix_waves = (int64_t)((p.n_mix + 3) / 4) * nC;
@@ -671,13 +673,79 @@ __global__ __launch_bounds__(256) void hyb_spmm_sparse_kernel(SpmmPlanDev p, con
     return;
   }
   const int64_t idx = wave - mix_waves;
-  if (idx >= (int64_t)(p.W - p.W_tc) * 4 * nC) return;
+  if (idx >= so_waves) return;
   const int w = p.W_tc + (int)(idx / (4 * nC));
   const int rest = (int)(idx - (int64_t)(w - p.W_tc) * 4 * nC);
   const int quarter = rest & 3;
   const int n0 = (rest >> 2) * 64;
   const int col = n0 + 4 * (lane & 15);
   finish_slot(p, B, ldb, N, C, ldc, col, vec, w * 16 + quarter * 4 + (lane >> 4), make_float4(0.f, 0.f, 0.f, 0.f));
+}
+
+// Sparse-only rows for launches with too few rows to fill the GPU at one row
+// per 16-lane group: one wave per (row, 64 columns), the row's nonzeros
+// dealt to the wave's four lane groups in turn (q - r0 mod 4), eight gathers
+// in flight per group; the four partial float4 are added by xor shuffles and
+// group 0 stores the row.  Rows with more than kHubNnz sparse nonzeros sum
+// with Neumaier compensation, as in finish_slot.
+__global__ __launch_bounds__(256) void hyb_spmm_sparse_split_kernel(SpmmPlanDev p, const float* __restrict__ B,
+                                                                    int64_t ldb, int N, float* __restrict__ C,
+                                                                    int64_t ldc, int nC, bool vec) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t rows = (int64_t)(p.W - p.W_tc) * 16;
+  if (wave >= rows * nC) return;
+  const int64_t ri = wave / nC;
+  const int n0 = (int)(wave - ri * nC) * 64;
+  const int slot = p.W_tc * 16 + (int)ri;
+  const int g = lane >> 4;
+  const int col = n0 + 4 * (lane & 15);
+  const int row = p.win_rows[slot];
+  const int r0 = p.res_ptr[slot], rend = p.res_ptr[slot + 1];
+  const int r1 = min(rend, r0 + kSplitNnz);
+  const bool compensated = rend - r0 > kHubNnz;
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f), cc = make_float4(0.f, 0.f, 0.f, 0.f);
+  // positions past r1 read the row's last entry with weight 0
+  for (int q0 = r0 + g; q0 < r1; q0 += 32) {
+    int k[8];
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int q = q0 + 4 * i;
+      const int qc = min(q, r1 - 1);
+      k[i] = p.res_cols[qc];
+      v[i] = q < r1 ? p.res_vals[qc] : 0.f;
+    }
+    float4 b[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b[i] = load_b4(B, ldb, k[i], col, N, vec);
+    if (compensated) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        neu_add(s.x, cc.x, v[i] * b[i].x);
+        neu_add(s.y, cc.y, v[i] * b[i].y);
+        neu_add(s.z, cc.z, v[i] * b[i].z);
+        neu_add(s.w, cc.w, v[i] * b[i].w);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        s.x += v[i] * b[i].x;
+        s.y += v[i] * b[i].y;
+        s.z += v[i] * b[i].z;
+        s.w += v[i] * b[i].w;
+      }
+    }
+  }
+  s.x += cc.x; s.y += cc.y; s.z += cc.z; s.w += cc.w;
+#pragma unroll
+  for (int off = 16; off <= 32; off <<= 1) {
+    s.x += __shfl_xor(s.x, off);
+    s.y += __shfl_xor(s.y, off);
+    s.z += __shfl_xor(s.z, off);
+    s.w += __shfl_xor(s.w, off);
+  }
+  if (g == 0 && row >= 0) store_c4(C, ldc, row, col, N, vec, s);
 }
 
 // B-stationary variant for BF16 plans with K <= kLdsMaxK: a workgroup stages
@@ -965,12 +1033,23 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  const int64_t sparse_waves = (int64_t)((p.n_mix + 3) / 4) * nC + (int64_t)(p.W - p.W_tc) * nC * 4;
+  // Sparse-only rows one per lane group, or, when that leaves fewer than 16
+  // waves per CU for rows of 8 or more nonzeros on average, one per wave.
+  const int64_t so_rows = (int64_t)(p.W - p.W_tc) * 16;
+  const bool split = so_rows > 0 && so_rows * nC / 4 < 4096 && p.nnz_sparse_only >= 8 * so_rows;
+  const int64_t sparse_waves = (int64_t)((p.n_mix + 3) / 4) * nC + (split ? 0 : so_rows * nC / 4);
+  if (split) {
+    const int64_t blocks = (so_rows * nC + 3) / 4;
+    hipLaunchKernelGGL(hyb_spmm_sparse_split_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, B, ldb, N, C,
+                       ldc, nC, vec);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   if (sparse_waves > 0) {
     const int64_t blocks = (sparse_waves + 3) / 4;
     if (blocks > INT32_MAX) return hipErrorInvalidValue;
     hipLaunchKernelGGL(hyb_spmm_sparse_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, B, ldb, N, C, ldc, nC,
-                       vec);
+                       vec, split ? (int64_t)0 : so_rows * nC / 4);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

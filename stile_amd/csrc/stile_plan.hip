@@ -566,6 +566,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.tc_vals = dptr<unsigned char>(plan, o_tcvals);
   d.tc_scale = tc_scale.empty() ? nullptr : dptr<float>(plan, o_scale);
   d.res_ptr = dptr<int32_t>(plan, o_resptr);
+  d.nnz_sparse_only = res_ptr[(size_t)W * 16] - res_ptr[(size_t)W_tc * 16];
   d.res_cols = dptr<int32_t>(plan, o_rescols);
   d.res_vals = dptr<float>(plan, o_resvals);
   plan->has_ids = bd.has_ids;

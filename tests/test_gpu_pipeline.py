@@ -163,3 +163,30 @@ def test_spmm_huge_windows_split_into_parts(cuda, blk):
     # a second run: the zeroing before the atomics makes runs independent
     C2 = plan.run_spmm(_t(B, cuda)).cpu().numpy()
     assert ops.max_rel_err(C2, ops.spmm_fp64(A, B)) <= tol
+
+
+@pytest.mark.parametrize("N", [64, 72, 70])
+def test_sparse_rows_split_over_lane_groups(cuda, N):
+    """Few sparse-only rows (the launch takes one wave per row): empty rows,
+    short and medium rows, rows longer than a window keeps (pieces added by
+    the fix-up kernel) and a hub row summed with compensation."""
+    rng = np.random.default_rng(11)
+    M, K = 512, 8192
+    lens = rng.choice([0, 3, 40, 300], size=M, p=[0.1, 0.3, 0.5, 0.1])
+    lens[7] = 5000
+    rows = np.repeat(np.arange(M), lens)
+    cols = np.concatenate([rng.choice(K, n, replace=False) for n in lens])
+    vals = rng.uniform(-1, 1, rows.size).astype(np.float32)
+    A = scipy.sparse.csr_matrix((vals, (rows, cols)), shape=(M, K))
+    B = rng.standard_normal((K, N), dtype=np.float32)
+    print(f"pipeline spmm split sparse rows N={N} nnz={A.nnz}", flush=True)
+    plan = api.build_spmm_plan(A, blk_dtype="float32", opts=api.SearchOptions(only_ELL=True), N=N)
+    assert plan.info()["block_nnz"] == 0
+    C = plan.run_spmm(_t(B, cuda)).cpu().numpy()
+    _check_fp32(C, A, B)
+    # bit-identical run to run, except rows with two or more fix-up pieces
+    # (more than 512 sparse nonzeros: their pieces are added atomically)
+    C2 = plan.run_spmm(_t(B, cuda)).cpu().numpy()
+    fixed = lens <= 512
+    assert np.array_equal(C[fixed], C2[fixed])
+    assert ops.max_rel_err(C2, C) <= 1e-6
