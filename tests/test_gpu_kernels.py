@@ -80,7 +80,7 @@ def test_csr_spmm_edges(cuda):
 
 @pytest.mark.parametrize("blk", ["float32", "bfloat16"])
 @pytest.mark.parametrize("layout", ["win1d", "rows2d", "k48"])
-@pytest.mark.parametrize("N", [64, 100, 256])
+@pytest.mark.parametrize("N", [64, 70, 100, 256])
 def test_hybrid_spmm_pruned(cuda, blk, layout, N):
     from stile_amd import _native
     A, _ = datasets.pruned_layer(seed=2, M=512, K=256, N=N)

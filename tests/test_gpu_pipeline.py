@@ -101,7 +101,7 @@ def test_spmm_pipeline_batched_uneven(cuda):
 
 
 @pytest.mark.parametrize("blk", ["float32", "bfloat16", "float16"])
-@pytest.mark.parametrize("N", [64, 100, 256])
+@pytest.mark.parametrize("N", [64, 70, 100, 256])
 def test_spmm_big_windows_split_over_a_workgroup(cuda, blk, N):
     """Windows with far more than 8 dense chunks (K = 2048, 5 % density, all
     TC tiles) run as workgroups that split the chunks over 8 waves."""
