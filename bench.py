@@ -132,7 +132,7 @@ def cpu_spmm_baseline(A, B, cols, seconds, label):
 
 def cpu_torch_csr_baseline(A, B, cols, seconds):
     """torch's CPU CSR x dense product on every host thread, reported beside
-    the one-core scipy baseline so the host side is not judged on one core."""
+    the one-core scipy baseline, so the host side is also measured on every core."""
     import warnings
     warnings.filterwarnings("ignore", message="Sparse CSR tensor support is in beta")
     import numpy as np

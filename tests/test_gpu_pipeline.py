@@ -4,7 +4,7 @@ kernels, against the FP64 oracle and the reference's FP32 CPU path.
 
 Bars: FP32 blocks -> within the FP32 dot-product error bound of FP64 and
 <= 1e-5 norm-wise relative difference to scipy FP32; BF16 blocks -> <= 1e-2
-norm-wise relative error vs FP64.
+norm-wise relative error vs FP64 (the stated bound for BF16 blocks).
 """
 import numpy as np
 import pytest
