@@ -205,6 +205,18 @@ def progress(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+
+
+def mfma_use(chunks, N, ms):
+    """Secondary figure for the dense blocks: MFMA work issued per launch
+    (every 16x32 chunk times N columns, padding slots included) over the
+    launch time, against the dense BF16 peak."""
+    tf = 2.0 * chunks * 16 * 32 * N / (ms * 1e-3) / 1e12
+    return {"issued_tflops": round(tf, 1), "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+            "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)}
+
+
 def _gf(flops, ms):
     return None if ms is None else round(flops / (ms * 1e-3) / 1e9, 2)
 
@@ -263,6 +275,7 @@ def run_pruned_layer(args, dev):
                hybrid_over_csr=round(ms_csr / ms, 3),
                hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),
                hybrid_over_sparsetir_hyb=round(ms_hyb / ms, 3),
+               mfma=mfma_use(info["num_chunks"], N, ms),
                n128=n128, launch_floor_us=round(ms_floor * 1e3, 2),
                hbm_copy_gbs={"value": round(copy_gbs, 1), "what": "256 MiB device copy (read + write)",
                              "achieved_over_copy": round(spmm_alg_bytes(A.nnz, M, K, N) / (ms * 1e-3) / 1e9
