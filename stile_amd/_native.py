@@ -179,19 +179,49 @@ def _dev_check(*ts):
     for t in ts:
         if not t.is_cuda:
             raise StileError("libstile_hip kernels take device tensors")
+    if len({t.device for t in ts}) > 1:
+        raise StileError("all tensors of a call must be on one device")
+
+
+def _i32(*ts):
+    import torch
+    for t in ts:
+        if t.dtype != torch.int32 or not t.is_contiguous():
+            raise StileError("index arrays must be contiguous int32")
+
+
+def _out(out, shape, device, what="out"):
+    """The output tensor of a call: a new one, or the caller's after checking
+    what the kernel will write through its pointer (float32, on `device`,
+    exactly `shape`, rows contiguous)."""
+    import torch
+    if out is None:
+        return torch.empty(shape, device=device, dtype=torch.float32)
+    if not out.is_cuda or out.device != device:
+        raise StileError(f"{what} must be on {device}")
+    if out.dtype != torch.float32:
+        raise StileError(f"{what} must be float32")
+    if tuple(out.shape) != tuple(shape):
+        raise StileError(f"{what} has shape {tuple(out.shape)}, expected {tuple(shape)}")
+    if out.numel() > 0 and out.stride(-1) != 1:
+        raise StileError(f"{what} must have unit stride along its last dimension")
+    return out
 
 
 def csr_spmm_f32(indptr, indices, vals, B, out=None):
     """C = A @ B for CSR A (int32 indptr/indices, float32 vals) on the device."""
     import torch
     _dev_check(indptr, indices, vals, B)
+    _i32(indptr, indices)
     M = indptr.numel() - 1
     K, N = B.shape
     if B.dtype != torch.float32 or vals.dtype != torch.float32:
         raise StileError("csr_spmm_f32 takes float32 values and B")
+    if vals.numel() != indices.numel() or not vals.is_contiguous():
+        raise StileError("vals must be contiguous with one value per index")
     if B.stride(1) != 1:
         B = B.contiguous()
-    C = out if out is not None else torch.empty((M, N), device=B.device, dtype=torch.float32)
+    C = _out(out, (M, N), B.device)
     _check(load().stile_csr_spmm_f32(M, K, N, indptr.data_ptr(), indices.data_ptr(), vals.data_ptr(),
                                      B.data_ptr(), B.stride(0), C.data_ptr(), C.stride(0), _stream(B)))
     return C
@@ -201,14 +231,19 @@ def csr_sddmm_f32(indptr, indices, X, Y, out=None):
     """out[p] = <X[row(p)], Y[indices[p]]> over the CSR pattern, on the device."""
     import torch
     _dev_check(indptr, indices, X, Y)
+    _i32(indptr, indices)
     M = indptr.numel() - 1
     nnz = indices.numel()
     Kf = X.shape[1]
+    if X.dtype != torch.float32 or Y.dtype != torch.float32:
+        raise StileError("csr_sddmm_f32 takes float32 X and Y")
+    if X.shape[0] < M or Y.shape[1] != Kf:
+        raise StileError("X must have a row per CSR row and Y as many features as X")
     if X.stride(1) != 1:
         X = X.contiguous()
     if Y.stride(1) != 1:
         Y = Y.contiguous()
-    o = out if out is not None else torch.empty(nnz, device=X.device, dtype=torch.float32)
+    o = _out(out, (nnz,), X.device)
     _check(load().stile_csr_sddmm_f32(M, Y.shape[0], nnz, Kf, indptr.data_ptr(), indices.data_ptr(),
                                       X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0),
                                       o.data_ptr(), _stream(X)))
@@ -218,7 +253,11 @@ def csr_sddmm_f32(indptr, indices, X, Y, out=None):
 def ell_buckets_spmm_f32(buckets, M, B, C):
     """C = sum of ELL-bucket products (buckets: list of device (row, cols,
     vals, width) tuples as made by stile_amd.hyb.HybSpMM)."""
+    import torch
     _dev_check(B, C)
+    if B.dtype != torch.float32 or B.stride(1) != 1:
+        raise StileError("B must be float32 with contiguous rows")
+    _out(C, (M, B.shape[1]), B.device, "C")
     arr = (StileEllBucket * max(len(buckets), 1))(*[
         StileEllBucket(rows=r.shape[0], width=w, row=r.data_ptr(), cols=c.data_ptr(), vals=v.data_ptr())
         for r, c, v, w in buckets])
@@ -231,7 +270,10 @@ def csr_row_softmax_f32(indptr, vals, scale=1.0, out=None):
     """Softmax of scale*vals over each CSR row (device float32)."""
     import torch
     _dev_check(indptr, vals)
-    o = out if out is not None else torch.empty_like(vals)
+    _i32(indptr)
+    if vals.dtype != torch.float32 or not vals.is_contiguous():
+        raise StileError("vals must be contiguous float32")
+    o = _out(out, tuple(vals.shape), vals.device)
     _check(load().stile_csr_row_softmax_f32(indptr.numel() - 1, indptr.data_ptr(), vals.data_ptr(),
                                             o.data_ptr(), float(scale), _stream(vals)))
     return o
@@ -347,7 +389,7 @@ class Plan:
                 raise StileError("Q, K, V must be bfloat16 of shape (rows, 64)")
         if not (Q.stride() == K.stride() == V.stride()) or Q.stride(1) != 1:
             Q, K, V = Q.contiguous(), K.contiguous(), V.contiguous()
-        o = out if out is not None else torch.empty((self.M, 64), device=Q.device, dtype=torch.float32)
+        o = _out(out, (self.M, 64), Q.device)
         _check(load().stile_attention_run(self._h, Q.data_ptr(), K.data_ptr(), V.data_ptr(), Q.stride(0), 64,
                                           float(scale), o.data_ptr(), o.stride(0), _stream(Q)))
         return o
@@ -392,7 +434,7 @@ class Plan:
         if B.stride(1) != 1:
             B = B.contiguous()
         N = B.shape[1]
-        C = out if out is not None else torch.empty((self.M, N), device=B.device, dtype=torch.float32)
+        C = _out(out, (self.M, N), B.device)
         _check(load().stile_spmm_run(self._h, B.data_ptr(), B.stride(0), N, C.data_ptr(), C.stride(0),
                                      _stream(B)))
         return C
@@ -404,9 +446,11 @@ class Plan:
         _dev_check(X, Y)
         if X.shape[0] != self.M or Y.shape[0] != self.K or X.shape[1] != Y.shape[1]:
             raise StileError("X/Y shapes do not match the plan")
+        if X.dtype != torch.float32 or Y.dtype != torch.float32:
+            raise StileError("X and Y must be float32")
         X = X.contiguous() if X.stride(1) != 1 else X
         Y = Y.contiguous() if Y.stride(1) != 1 else Y
-        o = out if out is not None else torch.empty(self.nnz, device=X.device, dtype=torch.float32)
+        o = _out(out, (self.nnz,), X.device)
         _check(load().stile_sddmm_run(self._h, X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0),
                                       X.shape[1], o.data_ptr(), _stream(X)))
         return o

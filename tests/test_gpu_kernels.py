@@ -14,7 +14,7 @@ import pytest
 import scipy.sparse
 
 from oracle import ops
-from stile_amd import datasets
+from stile_amd import api, datasets
 from tests import _packs
 
 pytestmark = pytest.mark.gpu
@@ -192,3 +192,25 @@ def test_hyb_ell_bucket_comparator(cuda, parts, N):
     C = h.run(_t(B, cuda)).cpu().numpy()
     ref64 = ops.spmm_fp64(A, B)
     assert ops.max_rel_err(C, ref64) <= 1e-5
+
+
+def test_output_and_index_arguments_are_checked(cuda):
+    """Caller-supplied outputs and index arrays are checked before their
+    pointers reach a kernel (a wrong shape or dtype raises, nothing runs)."""
+    import torch
+    from stile_amd import _native
+    A = datasets.uniform_spmm(seed=3, M=64, K=48, nnz=300, N=8)[0]
+    print("argument checks", flush=True)
+    plan = api.build_spmm_plan(A, blk_dtype="float32")
+    B = torch.randn(48, 8, device=cuda)
+    for bad in (torch.empty(63, 8, device=cuda), torch.empty(64, 8, device=cuda, dtype=torch.float64),
+                torch.empty(8, 64, device=cuda).t(), torch.empty(64, 8)):
+        with pytest.raises(_native.StileError):
+            plan.run_spmm(B, out=bad)
+    ip = torch.from_numpy(A.indptr.astype(np.int64)).to(cuda)
+    ix = torch.from_numpy(A.indices.astype(np.int32)).to(cuda)
+    vv = torch.from_numpy(A.data.astype(np.float32)).to(cuda)
+    with pytest.raises(_native.StileError):
+        _native.csr_spmm_f32(ip, ix, vv, B)
+    ok = _native.csr_spmm_f32(ip.int(), ix, vv, B, out=torch.empty(64, 8, device=cuda))
+    assert ops.max_rel_err(ok.cpu().numpy(), ops.spmm_fp64(A, B.cpu().numpy())) <= 1e-5
