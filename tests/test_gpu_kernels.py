@@ -214,3 +214,41 @@ def test_output_and_index_arguments_are_checked(cuda):
         _native.csr_spmm_f32(ip, ix, vv, B)
     ok = _native.csr_spmm_f32(ip.int(), ix, vv, B, out=torch.empty(64, 8, device=cuda))
     assert ops.max_rel_err(ok.cpu().numpy(), ops.spmm_fp64(A, B.cpu().numpy())) <= 1e-5
+
+
+def test_malformed_packs_are_rejected(cuda):
+    """Plan creation validates every index a kernel would follow: a column
+    past K, a row past M or a non-monotone window pointer raise StileError
+    (nothing is uploaded or launched)."""
+    import copy
+    from stile_amd import _native
+    A, _ = datasets.pruned_layer(seed=4, M=128, K=64, N=16)
+    pack = _packs.spmm_pack(A, min_fill=64)
+    assert pack["tc"] and pack["ell"]
+    print("malformed packs", flush=True)
+    _native.Plan.spmm(pack, "float32").close()  # the valid pack builds
+
+    def broken(edit):
+        p = copy.deepcopy(pack)
+        edit(p)
+        with pytest.raises(_native.StileError):
+            _native.Plan.spmm(p, "float32")
+
+    def ell_col(p):
+        g = p["ell"][0]
+        i = int(np.flatnonzero(np.asarray(g["a"]).reshape(-1) != 0)[0])
+        g["indices_k"].reshape(-1)[i] = A.shape[1]
+
+    def tc_col(p):
+        g = p["tc"][0]
+        t = int(np.flatnonzero(np.abs(np.asarray(g["a"])).sum(axis=1).reshape(len(g["indices"]), -1).sum(1))[0])
+        g["indices"][t] = A.shape[1] + 5
+    def tc_indptr(p):
+        g = p["tc"][0]
+        g["indptr"][1], g["indptr"][0] = g["indptr"][0] + len(g["indices"]) + 1, 0
+    def ell_row(p):
+        g = p["ell"][0]
+        g["indices_reordered_i"][0] = A.shape[0] + 1
+
+    for edit in (ell_col, tc_col, tc_indptr, ell_row):
+        broken(edit)
