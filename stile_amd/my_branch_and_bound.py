@@ -134,6 +134,8 @@ def _cover(tile, entries=None, full=False):
     tile.covered_rows = op._ent_row[e]
     tile.covered_cols = op._ent_col[e]
     op.state.uncovered[ids] = False
+    if tile.nnz == 0:  # made without an initial count (_make_tc_tile): what it covers
+        tile.nnz = len(ids)
     tile.nnz_when_selected = tile.j_num * len(ids)
     tile.nnz_uncovered = 0
     tile.set_avg_cost()

@@ -138,12 +138,19 @@ def set_params_BSPMM(formats, dsize, op=None):
 
 
 def set_params_BSDDMM(formats, dsize, op=None):
+    """Shape scalars per group (reference :2856-2968): a 1D group's NNZ is its
+    nonzero count padded to whole buckets of ty * group_size; a TC group has
+    MB = block rows, NB = block columns, NNZB = blocks, NNZ = nonzeros."""
     groups = []
     for fmt, tiles in formats.items():
         if fmt[0] == '1D_sddmm':
-            groups.append(dict(template=fmt[0], nnz=int(sum(t.nnz for t in tiles)), bucket=fmt[1][0]))
+            prm = json.loads(fmt[2])
+            bucket = prm['ty'] * prm['group_size']
+            nnz = int(sum(t.nnz for t in tiles))
+            groups.append(dict(template=fmt[0], nnz=-(-nnz // bucket) * bucket, nnz_real=nnz, bucket=bucket))
         else:
-            groups.append(dict(template=fmt[0], nnzb=len(tiles), tile_sizes=fmt[1]))
+            groups.append(dict(template=fmt[0], mb=int(fmt[1][0][1]), nb=int(fmt[1][2][1]), nnzb=len(tiles),
+                               nnz=int(sum(t.nnz for t in tiles)), tile_sizes=fmt[1]))
     return dict(groups=groups)
 
 
