@@ -154,3 +154,24 @@ def test_kvals_lazy_order_is_a_permutation():
     assert sorted(arr.tolist()) == list(range(12))
     assert arr[:3].tolist() == [5, 2, 9]
     assert np.array_equal(ko.pos(arr), np.arange(12))
+
+
+def test_compute_tile_known_answers():
+    """ComputeTile ranges and counts on the 6 x 8 matrix (N = 40): a TC tile
+    (16, 32, 16) at j-block 0 spans rows 0..5, columns 0..7, all 16 nonzeros,
+    32 output columns (uncovered work 16 * 32); at j-block 1 only 40 - 32 = 8
+    columns remain.  The ELL op (bucket 4) splits the two 5-nonzero rows, so it
+    has 8 rows; an (8, 32, 4) tile spans them all and holds every nonzero."""
+    A = tiny()
+    op = gf.Op('spmm', [0, 1], [2], [6, 40, 8], None, [A, gf.DenseTensor(np.zeros((8, 40), np.float32))], None)
+    gf.set_config(gf.SearchConfig())
+    ops, _ = gf.gen_candidate_formats(op, 4, ([1], [1], [1]), False, reorder_k_by_nnz=True)
+    ell, tc = ops[0], ops[-1]
+    ts = ((None, 16), (None, 32), (None, 16))
+    t = gf.ComputeTile(tc, ts, (0, 0, 0), {"mma_shape_str": "m16n16k16"})
+    assert (t.tile_i_rng, t.tile_k_rng, t.nnz, t.j_num, t.nnz_uncovered) == ((0, 5), (0, 7), 16, 32, 512)
+    t = gf.ComputeTile(tc, ts, (0, 1, 0), {"mma_shape_str": "m16n16k16"})
+    assert (t.nnz, t.j_num, t.nnz_uncovered) == (16, 8, 128)
+    assert len(ell.idx_values_list[0][0]) == 6 and ell.position_space[0].shape[0] == 8
+    te = gf.ComputeTile(ell, ((None, 8), (None, 32), (1, 4)), (0, 0, 0), {})
+    assert (te.tile_i_rng, te.tile_k_rng, te.nnz, te.j_num, te.nnz_uncovered) == ((0, 7), (0, 3), 16, 32, 512)
