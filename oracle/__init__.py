@@ -15,4 +15,12 @@ Contents
 
 Each function's docstring cites the reference file:line it follows and names
 the check that pins it (tests/test_oracle_*.py).
+
+Parity unpinned: the exact tile set the greedy search selects
+(my_search_formats.py:263-833 and the search functions of
+my_branch_and_bound.py) has no restatement here.  It depends on float
+tie-breaks and scipy internals, and the reference cannot be imported to
+produce golden tile lists.  Product packs are checked through invariants
+instead: every nonzero is covered once, re-expanding a pack reproduces A,
+and every SDDMM output id is produced.
 """
