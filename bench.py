@@ -44,6 +44,24 @@ def spmm_alg_bytes(nnz, M, K, N, s_b=4):
     return nnz * 8 + 4 * (M + 1) + K * N * s_b + M * N * 4
 
 
+def full_blocks(A, b=16):
+    """Number of completely dense b x b blocks of A (on the b-aligned grid)."""
+    import numpy as np
+    A = A.tocoo()
+    nbc = (A.shape[1] + b - 1) // b
+    key = (A.row // b).astype(np.int64) * nbc + A.col // b
+    _, counts = np.unique(key, return_counts=True)
+    return int(np.sum(counts == b * b))
+
+
+def spmm_alg_bytes_blocks(nblocks, nnz, M, K, N, s_blk=2, s_b=4):
+    """The dense-block variant: a dense 16 x 16 block costs its stored values
+    (256 * s_blk) and one 4-byte index; the other nonzeros value + index; then
+    indptr, B once and C once."""
+    residual = nnz - 256 * nblocks
+    return nblocks * (256 * s_blk + 4) + residual * 8 + 4 * (M + 1) + K * N * s_b + M * N * 4
+
+
 def sddmm_alg_bytes(nnz, M, Ncols, K):
     """index + output per nonzero, indptr, X and Y once."""
     return nnz * 8 + 4 * (M + 1) + (M + Ncols) * K * 4
@@ -244,6 +262,7 @@ def run_pruned_layer(args, dev):
     C_h = C.cpu().numpy()
     ms_csr, ms_vendor, ms_hyb = _spmm_comparators(A, Bd, max(args.steps // 4, 10), 5, hyb=True)
     flops = 2 * A.nnz * N
+    nblocks = full_blocks(A)
     n128 = None
     if not args.no_n128:
         B128 = torch.from_numpy(np.ascontiguousarray(B[:, :128])).to(dev)
@@ -253,7 +272,7 @@ def run_pruned_layer(args, dev):
         n128 = {"hybrid_us": round(ms128 * 1e3, 2), "csr_us": round(ms128_csr * 1e3, 2),
                 "vendor_us": None if ms128_vendor is None else round(ms128_vendor * 1e3, 2),
                 "hybrid_gflops": _gf(2 * A.nnz * 128, ms128),
-                "hbm_frac": roofline(spmm_alg_bytes(A.nnz, M, K, 128), ms128)["frac"]}
+                "hbm_frac": roofline(spmm_alg_bytes_blocks(nblocks, A.nnz, M, K, 128), ms128)["frac"]}
     # back-to-back launch floor (a one-element kernel): what a launch costs
     # before any work; it bounds small-N launches
     tiny = torch.zeros(1, device=dev)
@@ -269,7 +288,7 @@ def run_pruned_layer(args, dev):
                {"workload": "hybrid SpMM, block-pruned layer A 3072x768 (10% of 16x16 blocks + 1% residual, "
                             f"nnz={A.nnz}) x B 768x{N} FP32", "M": M, "K": K, "N": N, "nnz": int(A.nnz),
                 "global_batch": 1, "parallelism": "single GPU"},
-               roofline=roofline(spmm_alg_bytes(A.nnz, M, K, N), ms, traffic),
+               roofline=roofline(spmm_alg_bytes_blocks(nblocks, A.nnz, M, K, N), ms, traffic),
                plain_csr_gflops=_gf(flops, ms_csr), vendor_csr_gflops=_gf(flops, ms_vendor),
                sparsetir_hyb_gflops=_gf(flops, ms_hyb),
                hybrid_over_csr=round(ms_csr / ms, 3),
@@ -278,8 +297,8 @@ def run_pruned_layer(args, dev):
                mfma=mfma_use(info["num_chunks"], N, ms),
                n128=n128, launch_floor_us=round(ms_floor * 1e3, 2),
                hbm_copy_gbs={"value": round(copy_gbs, 1), "what": "256 MiB device copy (read + write)",
-                             "achieved_over_copy": round(spmm_alg_bytes(A.nnz, M, K, N) / (ms * 1e-3) / 1e9
-                                                         / copy_gbs, 4)},
+                             "achieved_over_copy": round(spmm_alg_bytes_blocks(nblocks, A.nnz, M, K, N)
+                                                         / (ms * 1e-3) / 1e9 / copy_gbs, 4)},
                plan={"windows": info["num_windows"], "mfma_chunks": info["num_chunks"],
                      "block_nnz": info["block_nnz"], "sparse_nnz": info["sparse_nnz"],
                      "search_tune_and_plan_s": round(t_plan, 3), "tuning": tuning})
