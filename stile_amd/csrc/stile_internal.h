@@ -27,6 +27,12 @@ constexpr int kBigWaves = 8;
 // one workgroup per window would leave its last wave running long after the
 // rest of the grid.  Windows up to kHugeChunks keep the fixed-order sums.
 constexpr int kHugeChunks = 256;
+// BF16 plans whose matrices all have K <= kLdsMaxK run their dense windows on
+// the B-stationary kernel (B rows of one matrix staged in LDS, K * 128 bytes).
+constexpr int kLdsMaxK = 1024;
+// Work units of the B-stationary kernel: lists for 2^0 .. 2^kLdsLists-1 units
+// (chosen per launch so that units x column chunks fill the GPU).
+constexpr int kLdsLists = 9;
 
 // Device view of a hybrid SpMM plan (window-major execution layout).
 //
@@ -68,6 +74,14 @@ struct SpmmPlanDev {
   int32_t n_mix;             // slots of dense windows with a sparse part
   const int32_t* mix_slot;   // [n_mix]
   const uint32_t* tc_mask;   // [nchunks][16] entry bits (attention plans), else nullptr
+  // B-stationary work units (BF16 plans with every matrix's K <= kLdsMaxK):
+  // unit = (first window, end window, first B row, B rows) of one matrix,
+  // windows in [0, W_tc).  List t (about 2^t units, chunk-balanced) starts at
+  // lds_units[lds_off[t]] and holds lds_cnt[t] units; lds_units is nullptr
+  // when the plan does not qualify.
+  const int4* lds_units;
+  int32_t lds_off[kLdsLists], lds_cnt[kLdsLists];
+  int32_t lds_max_k;         // largest B-row count of a unit
 };
 
 // Device view of a hybrid SDDMM plan.

@@ -749,12 +749,13 @@ __global__ __launch_bounds__(256) void hyb_spmm_sparse_split_kernel(SpmmPlanDev 
   if (g == 0 && row >= 0) store_c4(C, ldc, row, col, N, vec, s);
 }
 
-// B-stationary variant for BF16 plans with K <= kLdsMaxK: a workgroup stages
-// B[0:K, n0:n0+64] as BF16 in LDS (one 128-byte row per k, lane j's four
-// columns at byte 8j) and runs the dense windows [grp*wpg, (grp+1)*wpg) of
-// its column chunk against it, one window per wave at a time.  Each B row
-// then leaves L2 once per workgroup instead of once per window.
-constexpr int kLdsMaxK = 1024;
+// B-stationary variant for BF16 plans whose matrices all have K <= kLdsMaxK:
+// a workgroup takes one work unit (a run of dense windows of one matrix,
+// whose columns are B rows [koff, koff + ksz)) and one 64-column chunk,
+// stages B[koff:koff+ksz, n0:n0+64] as BF16 in LDS (one 128-byte row per k,
+// lane j's four columns at byte 8j) and runs the unit's windows against it,
+// a chunk-balanced run of windows per wave.  Each B row then leaves L2 once
+// per workgroup instead of once per window.
 
 __device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
   const uint16_t a = __builtin_bit_cast(uint16_t, (__bf16)lo);
@@ -819,33 +820,38 @@ __device__ __forceinline__ ChunkRegs load_chunk_regs(const SpmmPlanDev& p, int c
 
 // tc_ptr is p.tc_ptr, passed again as a restrict parameter so that the
 // compiler may read it with scalar loads inside the loop that stores C.
+// units: the work-unit list of this launch (unit = blockIdx.x / nC).
 __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p,
                                                                         const int32_t* __restrict__ tc_ptr,
+                                                                        const int4* __restrict__ units,
                                                                         const float* __restrict__ B,
                                                                         int64_t ldb, int N, float* __restrict__ C,
-                                                                        int64_t ldc, int nC, int wpg, bool vec) {
-  extern __shared__ uint2 bsh[];  // [K][16]: 4 BF16 per entry
+                                                                        int64_t ldc, int nC, bool vec) {
+  extern __shared__ uint2 bsh_all[];  // [ksz][16]: 4 BF16 per entry
   constexpr int kWaves = kLdsThreads / 64;
   const int chunk = blockIdx.x % nC;
-  const int grp = blockIdx.x / nC;
+  const int4 unit = units[blockIdx.x / nC];
   const int n0 = chunk * 64;
-  const int K = p.K;
-  // Stage B[0:K, n0:n0+64] as BF16: eight independent float4 loads per
-  // thread in flight before the packed stores.
-  for (int base = threadIdx.x; base < K * 16; base += 8 * kLdsThreads) {
+  const int koff = unit.z, ksz = unit.w;
+  // Stage B[koff:koff+ksz, n0:n0+64] as BF16: eight independent float4 loads
+  // per thread in flight before the packed stores.
+  for (int base = threadIdx.x; base < ksz * 16; base += 8 * kLdsThreads) {
     float4 v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int i = base + u * kLdsThreads;
-      v[u] = i < K * 16 ? load_b4(B, ldb, i >> 4, n0 + 4 * (i & 15), N, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
+      v[u] = i < ksz * 16 ? load_b4(B, ldb, koff + (i >> 4), n0 + 4 * (i & 15), N, vec)
+                          : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int i = base + u * kLdsThreads;
-      if (i < K * 16) bsh[i] = make_uint2(bf16_pair(v[u].x, v[u].y), bf16_pair(v[u].z, v[u].w));
+      if (i < ksz * 16) bsh_all[i] = make_uint2(bf16_pair(v[u].x, v[u].y), bf16_pair(v[u].z, v[u].w));
     }
   }
   __syncthreads();
+  // the unit's column ids k lie in [koff, koff + ksz): LDS row k - koff
+  const uint2* bsh = bsh_all - (int64_t)koff * 16;
   // The wave index is made visibly uniform so that the window and chunk
   // bookkeeping below lives in scalar registers and reads tc_ptr with scalar
   // loads: a vector load of the next window bound inside the loop would make
@@ -853,9 +859,9 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = lane >> 4, j = lane & 15;
   const int col = n0 + 4 * j;
-  // This wave's contiguous run of windows, balanced by chunk count.
-  const int wg_beg = grp * wpg;
-  const int wg_end = min(wg_beg + wpg, p.W_tc);
+  // This wave's contiguous run of the unit's windows, balanced by chunk count.
+  const int wg_beg = unit.x;
+  const int wg_end = unit.y;
   if (wg_beg >= wg_end) return;
   const int cb = tc_ptr[wg_beg], ce = tc_ptr[wg_end];
   auto window_at = [&](int target) {  // first window w in [wg_beg, wg_end] with tc_ptr[w] >= target
@@ -970,9 +976,9 @@ __global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, cons
 }
 
 void prepare_spmm_kernels(const SpmmPlanDev& p) {
-  if (p.blk_dtype == 1 && p.K <= kLdsMaxK && p.K * 128 > 65536)
+  if (p.lds_units && p.lds_max_k * 128 > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, p.K * 128);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, p.lds_max_k * 128);
 }
 
 // STILE_SPMM_KERNEL=stream forces the streaming dense kernel for BF16 plans
@@ -993,21 +999,23 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
   const bool vec = (N % 4 == 0) && (ldb % 4 == 0) && (ldc % 4 == 0) &&
                    ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && ((reinterpret_cast<uintptr_t>(C) & 15) == 0);
   const int nC = (N + 63) / 64;
-  // B-stationary kernel for every BF16 plan with K <= kLdsMaxK (measured on
-  // the block-pruned layer: 9.0 us vs 15.0 us streaming at N = 128, 20.8 vs
-  // 82 us at N = 4096); otherwise the streaming kernel.  Window groups per
-  // column chunk: enough workgroups for every CU.
-  const int groups = std::max(1, (256 + nC - 1) / nC);
-  bool lds = p.blk_dtype == 1 && p.K <= kLdsMaxK && p.W_tc > 0;
+  // B-stationary kernel for every BF16 plan whose matrices all have K <=
+  // kLdsMaxK (measured on the block-pruned layer: 9.0 us vs 15.0 us
+  // streaming at N = 128, 20.8 vs 82 us at N = 4096); otherwise the
+  // streaming kernel.  Unit list: the smallest with at least 256 / nC units
+  // (enough workgroups for every CU), or the largest.
+  bool lds = p.lds_units != nullptr && p.W_tc > 0;
   if (spmm_kernel_choice() == 1) lds = false;
   // The LDS kernel runs every dense window; otherwise the streaming kernel
   // does (its first blocks the big windows).  The sparse-only windows follow.
   if (lds) {
-    const int g = std::min(groups, p.W_tc);
-    const int wpg = (p.W_tc + g - 1) / g;
-    const int grid = nC * ((p.W_tc + wpg - 1) / wpg);
-    hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel, dim3(grid), dim3(kLdsThreads), (size_t)p.K * 128, s, p,
-                       p.tc_ptr, B, ldb, N, C, ldc, nC, wpg, vec);
+    int t = 0;
+    while (t + 1 < kLdsLists && (int64_t)p.lds_cnt[t] * nC < 256) ++t;
+    const int64_t grid = (int64_t)p.lds_cnt[t] * nC;
+    if (grid > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel, dim3((unsigned)grid), dim3(kLdsThreads),
+                       (size_t)p.lds_max_k * 128, s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C,
+                       ldc, nC, vec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   } else if (p.W_tc > 0) {

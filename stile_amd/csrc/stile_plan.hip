@@ -130,6 +130,8 @@ struct SpmmBuild {
   bool has_ids = true;     // every group carried a_ids
   bool want_mask = false;  // also build tc_mask (fused attention)
   int64_t max_id = -1;     // largest value id (values array must be longer)
+  // block-diagonal batches: first row / column of each matrix, then M / K
+  std::vector<int64_t> inst_rows, inst_cols;
 };
 
 // Validate one pack and append its windows / products to the build with row
@@ -308,6 +310,60 @@ int add_spmm_pack(SpmmBuild& bd, const StileSpmmPack* pk, int32_t roff, int32_t 
   return STILE_OK;
 }
 
+// Matrix (batch index) of window w: the one holding its first row.
+int32_t window_matrix(const SpmmBuild& bd, int32_t w) {
+  if (bd.inst_rows.size() <= 2) return 0;
+  for (int r = 0; r < 16; ++r) {
+    const int32_t row = bd.win_rows[(size_t)w * 16 + r];
+    if (row >= 0)
+      return (int32_t)(std::upper_bound(bd.inst_rows.begin(), bd.inst_rows.end(), (int64_t)row) -
+                       bd.inst_rows.begin()) - 1;
+  }
+  return 0;
+}
+
+// Work-unit lists of the B-stationary kernel.  The dense windows [0, W_tc)
+// are grouped by matrix (contiguous); list t splits each matrix's windows
+// into about 2^t * (its share of the chunks) contiguous, chunk-balanced units
+// (at least one per matrix).  Appends the lists to `units`.
+void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, int32_t W_tc,
+                     std::vector<int32_t>& units, int32_t (&off)[stile::kLdsLists],
+                     int32_t (&cnt)[stile::kLdsLists], int32_t& max_k) {
+  // runs of windows of one matrix
+  std::vector<int32_t> run_w0, run_m;
+  for (int32_t w = 0; w < W_tc; ++w) {
+    const int32_t m = window_matrix(bd, w);
+    if (run_m.empty() || run_m.back() != m) {
+      run_w0.push_back(w);
+      run_m.push_back(m);
+    }
+  }
+  run_w0.push_back(W_tc);
+  const int64_t total = tc_ptr[W_tc];
+  max_k = 0;
+  for (int t = 0; t < stile::kLdsLists; ++t) {
+    off[t] = (int32_t)(units.size() / 4);
+    const int64_t target = (int64_t)1 << t;
+    for (size_t r = 0; r + 1 < run_w0.size(); ++r) {
+      const int32_t w0 = run_w0[r], w1 = run_w0[r + 1], m = run_m[r];
+      const int64_t c0 = tc_ptr[w0], chunks = tc_ptr[w1] - c0;
+      const int64_t n = std::max<int64_t>(1, std::min<int64_t>(w1 - w0, (target * chunks + total - 1) / std::max<int64_t>(total, 1)));
+      const int32_t koff = (int32_t)bd.inst_cols[m], ksz = (int32_t)(bd.inst_cols[m + 1] - bd.inst_cols[m]);
+      max_k = std::max(max_k, ksz);
+      int32_t a = w0;
+      for (int64_t u = 1; u <= n && a < w1; ++u) {
+        // end: first window whose chunk start reaches u / n of the run
+        const int64_t goal = c0 + (chunks * u + n - 1) / n;
+        int32_t b = u == n ? w1 : a + 1;
+        while (b < w1 && tc_ptr[b] < goal) ++b;
+        units.insert(units.end(), {a, b, koff, ksz});
+        a = b;
+      }
+    }
+    cnt[t] = (int32_t)(units.size() / 4) - off[t];
+  }
+}
+
 int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const int32_t W = (int32_t)(bd.win_rows.size() / 16);
 
@@ -341,12 +397,26 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   std::stable_partition(order.begin(), order.end(), [&](int32_t w) { return !bd.win_cols[w].empty(); });
   int32_t W_tc = 0;
   while (W_tc < W && !bd.win_cols[order[W_tc]].empty()) ++W_tc;
-  auto big_end = std::stable_partition(order.begin(), order.begin() + W_tc,
-                                       [&](int32_t w) { return nch(w) > stile::kBigChunks; });
-  std::stable_sort(order.begin(), big_end, [&](int32_t a, int32_t b) { return nch(a) > nch(b); });
-  const int32_t W_big = (int32_t)(big_end - order.begin());
-  int32_t W_huge = 0;
-  while (W_huge < W_big && nch(order[W_huge]) > stile::kHugeChunks) ++W_huge;
+  // BF16 plans whose matrices all have K <= kLdsMaxK run on the B-stationary
+  // kernel, which stages one matrix's B rows per workgroup: in a batch of
+  // several matrices their dense windows are kept grouped by matrix (and no
+  // window counts as big or huge, which only the streaming kernel uses).
+  int64_t max_k = 0;
+  for (size_t i = 0; i + 1 < bd.inst_cols.size(); ++i) max_k = std::max(max_k, bd.inst_cols[i + 1] - bd.inst_cols[i]);
+  const bool lds_ok = blk_dtype == STILE_BLK_BF16 && max_k <= stile::kLdsMaxK && !bd.want_mask;
+  const bool by_matrix = lds_ok && bd.inst_rows.size() > 2;
+  int32_t W_big = 0, W_huge = 0;
+  if (by_matrix) {
+    std::vector<int32_t> mat(W);
+    for (int32_t w = 0; w < W; ++w) mat[w] = window_matrix(bd, w);
+    std::stable_sort(order.begin(), order.begin() + W_tc, [&](int32_t a, int32_t b) { return mat[a] < mat[b]; });
+  } else {
+    auto big_end = std::stable_partition(order.begin(), order.begin() + W_tc,
+                                         [&](int32_t w) { return nch(w) > stile::kBigChunks; });
+    std::stable_sort(order.begin(), big_end, [&](int32_t a, int32_t b) { return nch(a) > nch(b); });
+    W_big = (int32_t)(big_end - order.begin());
+    while (W_huge < W_big && nch(order[W_huge]) > stile::kHugeChunks) ++W_huge;
+  }
   {
     std::vector<int32_t> new_of(W);
     for (int32_t nw = 0; nw < W; ++nw) new_of[order[nw]] = nw;
@@ -425,6 +495,9 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
     tc_ptr[w + 1] = (int32_t)nchunks;
   }
   if (nchunks > INT32_MAX / 64) return fail(STILE_E_UNSUPPORTED, "too many dense chunks");
+  std::vector<int32_t> lds_units;
+  int32_t lds_off[stile::kLdsLists] = {}, lds_cnt[stile::kLdsLists] = {}, lds_max_k = 0;
+  if (lds_ok && W_tc > 0) build_lds_units(bd, tc_ptr, W_tc, lds_units, lds_off, lds_cnt, lds_max_k);
   // Parts of the huge windows: consecutive ranges of at most kHugeChunks chunks.
   std::vector<int32_t> part_win, part_c0, part_c1;
   for (int32_t w = 0; w < W_huge; ++w)
@@ -509,6 +582,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const size_t o_pwin = lay.add(part_win.size() * 4);
   const size_t o_pc0 = lay.add(part_c0.size() * 4);
   const size_t o_pc1 = lay.add(part_c1.size() * 4);
+  const size_t o_units = lay.add(lds_units.size() * 4);
   HostBlob blob;
   blob.data.assign(lay.bytes, 0);
   blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
@@ -529,6 +603,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   blob.put(o_pwin, part_win.data(), part_win.size() * 4);
   blob.put(o_pc0, part_c0.data(), part_c0.size() * 4);
   blob.put(o_pc1, part_c1.data(), part_c1.size() * 4);
+  blob.put(o_units, lds_units.data(), lds_units.size() * 4);
 
   StilePlan* plan = new (std::nothrow) StilePlan();
   if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
@@ -554,6 +629,12 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.n_mix = (int32_t)mix_slot.size();
   d.mix_slot = dptr<int32_t>(plan, o_mix);
   d.tc_mask = tc_mask.empty() ? nullptr : dptr<uint32_t>(plan, o_mask);
+  d.lds_units = lds_units.empty() ? nullptr : dptr<int4>(plan, o_units);
+  for (int t = 0; t < stile::kLdsLists; ++t) {
+    d.lds_off[t] = lds_off[t];
+    d.lds_cnt[t] = lds_cnt[t];
+  }
+  d.lds_max_k = lds_max_k;
   d.nchunks = (int32_t)nchunks;
   d.npieces = (int32_t)piece_row.size();
   d.piece_row = dptr<int32_t>(plan, o_prow);
@@ -609,6 +690,8 @@ int create_spmm(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype, St
     // ones: the batched values array is the concatenation of the matrices'
     // value arrays, each as long as its largest id + 1
     ioff = bd.max_id + 1;
+    bd.inst_rows.push_back(roff);
+    bd.inst_cols.push_back(coff);
     int rc = add_spmm_pack(bd, &packs[i], (int32_t)roff, (int32_t)coff, ioff);
     if (rc != STILE_OK) return rc;
     (void)w0;
@@ -619,6 +702,8 @@ int create_spmm(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype, St
   }
   bd.M = (int32_t)roff;
   bd.K = (int32_t)coff;
+  bd.inst_rows.push_back(roff);
+  bd.inst_cols.push_back(coff);
   return finish_spmm(bd, blk_dtype, out);
 }
 

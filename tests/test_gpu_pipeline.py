@@ -190,3 +190,24 @@ def test_sparse_rows_split_over_lane_groups(cuda, N):
     fixed = lens <= 512
     assert np.array_equal(C[fixed], C2[fixed])
     assert ops.max_rel_err(C2, C) <= 1e-6
+
+
+@pytest.mark.parametrize("N", [128, 70, 512])
+def test_batched_bf16_matrices_on_the_b_stationary_kernel(cuda, N):
+    """Uneven block-pruned matrices (K <= 1024 each) in one block-diagonal
+    BF16 plan: every workgroup stages the B rows of one matrix.  Each
+    matrix's rows of C vs FP64 (1e-2), and run to run bit-identical."""
+    import torch
+    shapes = [(512, 256), (1024, 1024), (256, 768), (768, 768), (96, 1024)]
+    mats = [datasets.pruned_layer(seed=20 + i, M=m, K=k, N=8)[0] for i, (m, k) in enumerate(shapes)]
+    print(f"batched bf16 B-stationary N={N} mats={len(mats)}", flush=True)
+    plan = api.build_spmm_plan_batched(mats, blk_dtype="bfloat16", N=N)
+    Ktot = sum(k for _, k in shapes)
+    B = np.random.default_rng(5).standard_normal((Ktot, N), dtype=np.float32)
+    C = plan.run_spmm(_t(B, cuda)).cpu().numpy()
+    r0 = c0 = 0
+    for A, (m, k) in zip(mats, shapes):
+        rel = ops.max_rel_err(C[r0:r0 + m], ops.spmm_fp64(A, B[c0:c0 + k]))
+        assert rel <= 1e-2, f"matrix {m}x{k}: {rel:.3g}"
+        r0, c0 = r0 + m, c0 + k
+    assert np.array_equal(C, plan.run_spmm(_t(B, cuda)).cpu().numpy())
