@@ -9,6 +9,9 @@ Default workload (one step = one launch of the fused SpMM):
   accumulation, the sparse rest FP32.
 
 Other workloads (--workload), each printed as its own JSON line:
+  pruned_batch  120 block-pruned matrices of a 24-layer BERT-large-shaped
+                encoder (Q, K, V, O 1024 x 1024, FFN-in 4096 x 1024; 21.95M
+                nonzeros) x B 1024 x 128 each, in one block-diagonal launch
   uniform_spmm  A 4096^2, 167,772 uniform nonzeros U(-1,1), B 4096 x 64 FP32
   rmat_sddmm    R-MAT scale 16 (+ self loops), X, Y 65536 x 32 FP32
   gcn           GCN layer on R-MAT scale 20 (32.45M nonzeros, symmetric
@@ -309,6 +312,64 @@ def run_pruned_layer(args, dev):
     return out
 
 
+def run_pruned_batch(args, dev):
+    """Many uneven BF16-block matrices in one launch at N = 128: the
+    block-pruned projections of a 24-layer BERT-large-shaped encoder (120
+    matrices, K = 1024 each), one block-diagonal plan on the B-stationary
+    kernel."""
+    import numpy as np
+    import scipy.sparse
+    import torch
+    from oracle import ops
+    from stile_amd import api, datasets
+    N = 128
+    t0 = time.perf_counter()
+    mats, shapes = datasets.pruned_encoder()
+    progress(f"pruned_batch: {len(mats)} matrices, nnz {sum(A.nnz for A in mats)}")
+    plan, _, tuning = api.tune_plan("spmm", mats, "bfloat16", N, log=progress,
+                                    cache=cache_path(args, "pruned_batch_n128"))
+    t_plan = time.perf_counter() - t0
+    info = plan.info()
+    Mt, Kt = sum(m for m, _ in shapes), sum(k for _, k in shapes)
+    nnz = sum(A.nnz for A in mats)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    Bd = torch.randn(Kt, N, device=dev, generator=g)
+    C = torch.empty((Mt, N), device=dev, dtype=torch.float32)
+    ms = time_launches(lambda: plan.run_spmm(Bd, C), args.steps, args.warmup)
+    big = scipy.sparse.block_diag(mats, format="csr").astype(np.float32)
+    ms_csr, ms_vendor = _spmm_comparators(big, Bd, max(args.steps // 4, 10), 3)
+    alg = sum(spmm_alg_bytes_blocks(full_blocks(A), A.nnz, m, k, N) for A, (m, k) in zip(mats, shapes))
+    flops = 2 * nnz * N
+    out = line("SpMM GFLOP/s (2*nnz*N), many BF16-block matrices in one launch at N=128",
+               flops / (ms * 1e-3) / 1e9, "GFLOP/s", ms, args, "bf16 MFMA blocks + f32 sparse, f32 accumulate",
+               {"workload": f"{len(mats)} block-pruned projections of a 24-layer BERT-large-shaped encoder "
+                            f"(Q, K, V, O 1024x1024 and FFN-in 4096x1024 per layer; 10% of 16x16 blocks + 1% "
+                            f"residual; nnz={nnz}) x B 1024x128 each, one block-diagonal launch",
+                "matrices": len(mats), "M_total": Mt, "K_total": Kt, "N": N, "nnz": int(nnz),
+                "parallelism": "single GPU"},
+               roofline=roofline(alg, ms),
+               plain_csr_gflops=_gf(flops, ms_csr), vendor_csr_gflops=_gf(flops, ms_vendor),
+               hybrid_over_csr=round(ms_csr / ms, 3),
+               hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),
+               mfma=mfma_use(info["num_chunks"], N, ms),
+               plan={"windows": info["num_windows"], "mfma_chunks": info["num_chunks"],
+                     "block_nnz": info["block_nnz"], "sparse_nnz": info["sparse_nnz"],
+                     "search_tune_and_plan_s": round(t_plan, 3), "tuning": tuning})
+    if not args.no_cpu:
+        Bh = Bd.cpu().numpy()
+        Ch = C.cpu().numpy()
+        errs = []
+        r0 = c0 = 0
+        for i, (A, (m, k)) in enumerate(zip(mats, shapes)):
+            if i in (0, 4, len(mats) - 1):
+                errs.append(ops.max_rel_err(Ch[r0:r0 + m], ops.spmm_fp64(A, Bh[c0:c0 + k])))
+            r0, c0 = r0 + m, c0 + k
+        out["max_rel_err_vs_fp64_3_matrices"] = max(errs)
+        out["cpu_baseline"] = cpu_spmm_baseline(mats[4], Bh[4 * 1024:5 * 1024], N, 10.0, "4096x1024 FFN-in")
+    return out
+
+
 def run_uniform_spmm(args, dev):
     import torch
     from oracle import ops
@@ -538,7 +599,8 @@ def run_attention(args, dev):
 
 
 WORKLOADS = {"pruned_layer": run_pruned_layer, "uniform_spmm": run_uniform_spmm,
-             "rmat_sddmm": run_rmat_sddmm, "gcn": run_gcn, "attention": run_attention}
+             "rmat_sddmm": run_rmat_sddmm, "gcn": run_gcn, "attention": run_attention,
+             "pruned_batch": run_pruned_batch}
 
 
 def main():
@@ -562,7 +624,7 @@ def main():
     dev = torch.device("cuda:0")
     names = [args.workload]
     if args.workload == "all":
-        names = ["pruned_layer", "uniform_spmm", "rmat_sddmm", "attention", "gcn"]
+        names = ["pruned_layer", "pruned_batch", "uniform_spmm", "rmat_sddmm", "attention", "gcn"]
     for name in names:
         print(json.dumps(WORKLOADS[name](args, dev)), flush=True)
 

@@ -30,9 +30,12 @@ constexpr int kHugeChunks = 256;
 // BF16 plans whose matrices all have K <= kLdsMaxK run their dense windows on
 // the B-stationary kernel (B rows of one matrix staged in LDS, K * 128 bytes).
 constexpr int kLdsMaxK = 1024;
-// Work units of the B-stationary kernel: lists for 2^0 .. 2^kLdsLists-1 units
-// (chosen per launch so that units x column chunks fill the GPU).
-constexpr int kLdsLists = 9;
+// Work units of the B-stationary kernel: kLdsLists lists of about
+// kLdsTargets[t] units each; the plan picks one per column-chunk count
+// nC <= kLdsPickMax by a makespan estimate (see build_lds_units).
+constexpr int kLdsLists = 12;
+constexpr int kLdsTargets[kLdsLists] = {1, 2, 4, 8, 16, 32, 64, 128, 192, 256, 384, 512};
+constexpr int kLdsPickMax = 64;
 
 // Device view of a hybrid SpMM plan (window-major execution layout).
 //
@@ -82,6 +85,7 @@ struct SpmmPlanDev {
   const int4* lds_units;
   int32_t lds_off[kLdsLists], lds_cnt[kLdsLists];
   int32_t lds_max_k;         // largest B-row count of a unit
+  int8_t lds_pick[kLdsPickMax + 1];  // list to run for nC column chunks
 };
 
 // Device view of a hybrid SDDMM plan.
@@ -104,6 +108,9 @@ struct SddmmPlanDev {
 };
 
 // Kernel launchers (stile_kernels.hip).  All return hipError_t of the launch.
+// Compute units of the current device (queried once per device).
+int device_cus();
+
 hipError_t launch_csr_spmm_f32(int M, int N, const int32_t* indptr,
                                const int32_t* indices, const float* vals,
                                const float* B, int64_t ldb, float* C,

@@ -818,19 +818,19 @@ __device__ __forceinline__ ChunkRegs load_chunk_regs(const SpmmPlanDev& p, int c
   return r;
 }
 
-// tc_ptr is p.tc_ptr, passed again as a restrict parameter so that the
-// compiler may read it with scalar loads inside the loop that stores C.
-// units: the work-unit list of this launch (unit = blockIdx.x / nC).
-__global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p,
-                                                                        const int32_t* __restrict__ tc_ptr,
-                                                                        const int4* __restrict__ units,
-                                                                        const float* __restrict__ B,
-                                                                        int64_t ldb, int N, float* __restrict__ C,
-                                                                        int64_t ldc, int nC, bool vec) {
-  extern __shared__ uint2 bsh_all[];  // [ksz][16]: 4 BF16 per entry
+// One work item (unit item / nC, column chunk item % nC) of the B-stationary
+// kernel.  tc_ptr is p.tc_ptr, passed again as a restrict parameter so that
+// the compiler may read it with scalar loads inside the loop that stores C.
+// Returns early per wave (a wave with no windows) or per workgroup (an empty
+// unit); the caller's barrier before the next item's staging keeps the LDS
+// block until every wave is done with it.
+__device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __restrict__ tc_ptr,
+                                         const int4* __restrict__ units, const float* __restrict__ B,
+                                         int64_t ldb, int N, float* __restrict__ C, int64_t ldc, int nC,
+                                         bool vec, int item, uint2* __restrict__ bsh_all) {
   constexpr int kWaves = kLdsThreads / 64;
-  const int chunk = blockIdx.x % nC;
-  const int4 unit = units[blockIdx.x / nC];
+  const int chunk = item % nC;
+  const int4 unit = units[item / nC];
   const int n0 = chunk * 64;
   const int koff = unit.z, ksz = unit.w;
   // Stage B[koff:koff+ksz, n0:n0+64] as BF16: eight independent float4 loads
@@ -862,7 +862,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   // This wave's contiguous run of the unit's windows, balanced by chunk count.
   const int wg_beg = unit.x;
   const int wg_end = unit.y;
-  if (wg_beg >= wg_end) return;
+  if (wg_beg >= wg_end) return;  // the whole workgroup
   const int cb = tc_ptr[wg_beg], ce = tc_ptr[wg_end];
   auto window_at = [&](int target) {  // first window w in [wg_beg, wg_end] with tc_ptr[w] >= target
     int lo = wg_beg, hi = wg_end;
@@ -917,6 +917,30 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   if (c < c_end) step(r0, r2);
   if (c < c_end) step(r1, r0);
   store_window<true>(C, ldc, N, col, vec, sr, acc);
+}
+
+// units: the work-unit list of this launch, largest units first.  kLoop =
+// false: one workgroup per item (grid = items <= compute units); true:
+// persistent workgroups (grid = compute units) take items i = blockIdx.x,
+// blockIdx.x + gridDim.x, ... and restage the LDS block for each.  Two
+// instantiations because the loop costs the one-item launches about 0.6 us
+// (more registers live across the item body).
+template <bool kLoop>
+__global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p,
+                                                                        const int32_t* __restrict__ tc_ptr,
+                                                                        const int4* __restrict__ units,
+                                                                        const float* __restrict__ B,
+                                                                        int64_t ldb, int N, float* __restrict__ C,
+                                                                        int64_t ldc, int nC, int n_items, bool vec) {
+  extern __shared__ uint2 bsh_all[];  // [ksz][16]: 4 BF16 per entry
+  if (!kLoop) {
+    lds_item(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, blockIdx.x, bsh_all);
+    return;
+  }
+  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    __syncthreads();  // every wave is done reading the previous item's block
+    lds_item(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, item, bsh_all);
+  }
 }
 
 // Remaining pieces of long sparse rows: lane group g of the wave takes
@@ -977,8 +1001,21 @@ __global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, cons
 
 void prepare_spmm_kernels(const SpmmPlanDev& p) {
   if (p.lds_units && p.lds_max_k * 128 > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, p.lds_max_k * 128);
+    for (const void* f : {reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<false>),
+                          reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<true>)})
+      (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, p.lds_max_k * 128);
+}
+
+int device_cus() {
+  static int cus[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
 }
 
 // STILE_SPMM_KERNEL=stream forces the streaming dense kernel for BF16 plans
@@ -993,6 +1030,18 @@ static int spmm_kernel_choice() {
   return choice;
 }
 
+// STILE_LDS_LIST=t forces unit list t of the B-stationary kernel (for launch
+// shape sweeps); unset or out of range: the plan's choice for the column
+// chunk count.  Read once.
+static int lds_list_override() {
+  static const int t = [] {
+    const char* e = getenv("STILE_LDS_LIST");
+    const int v = e ? atoi(e) : -1;
+    return (v >= 0 && v < kLdsLists) ? v : -1;
+  }();
+  return t;
+}
+
 hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, int N, float* C, int64_t ldc,
                            hipStream_t s) {
   if (p.W == 0 || N == 0) return hipSuccess;
@@ -1002,20 +1051,30 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
   // B-stationary kernel for every BF16 plan whose matrices all have K <=
   // kLdsMaxK (measured on the block-pruned layer: 9.0 us vs 15.0 us
   // streaming at N = 128, 20.8 vs 82 us at N = 4096); otherwise the
-  // streaming kernel.  Unit list: the smallest with at least 256 / nC units
-  // (enough workgroups for every CU), or the largest.
+  // streaming kernel.  Unit list: the plan's pick for nC column chunks (up to
+  // kLdsPickMax), else the smallest list with at least one item per CU.
   bool lds = p.lds_units != nullptr && p.W_tc > 0;
   if (spmm_kernel_choice() == 1) lds = false;
   // The LDS kernel runs every dense window; otherwise the streaming kernel
   // does (its first blocks the big windows).  The sparse-only windows follow.
   if (lds) {
-    int t = 0;
-    while (t + 1 < kLdsLists && (int64_t)p.lds_cnt[t] * nC < 256) ++t;
-    const int64_t grid = (int64_t)p.lds_cnt[t] * nC;
-    if (grid > INT32_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel, dim3((unsigned)grid), dim3(kLdsThreads),
-                       (size_t)p.lds_max_k * 128, s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C,
-                       ldc, nC, vec);
+    const int cus = device_cus();
+    int t = lds_list_override();
+    if (t < 0 && nC <= kLdsPickMax) t = p.lds_pick[nC];
+    if (t < 0) {
+      t = 0;
+      while (t + 1 < kLdsLists && (int64_t)p.lds_cnt[t] * nC < cus) ++t;
+    }
+    const int64_t items = (int64_t)p.lds_cnt[t] * nC;
+    if (items > INT32_MAX) return hipErrorInvalidValue;
+    // one workgroup per CU at most (each holds a staged block of LDS)
+    const int grid = (int)std::min<int64_t>(items, cus);
+    if (items <= cus)
+      hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel<false>, dim3(grid), dim3(kLdsThreads), (size_t)p.lds_max_k * 128,
+                         s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C, ldc, nC, (int)items, vec);
+    else
+      hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel<true>, dim3(grid), dim3(kLdsThreads), (size_t)p.lds_max_k * 128,
+                         s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C, ldc, nC, (int)items, vec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   } else if (p.W_tc > 0) {

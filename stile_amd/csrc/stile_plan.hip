@@ -15,6 +15,7 @@
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -322,13 +323,35 @@ int32_t window_matrix(const SpmmBuild& bd, int32_t w) {
   return 0;
 }
 
+// Estimated time of one workgroup item of the B-stationary kernel in units
+// of one 16x32 chunk times 64 columns: the chunks plus staging ksz B rows,
+// which costs about one chunk per kLdsStageRows rows.
+constexpr int64_t kLdsStageRows = 8;
+
+// Longest workgroup of a launch of `items` items (unit i / nC, sorted by
+// decreasing size) over min(items, cus) persistent workgroups that take
+// items round-robin.
+int64_t lds_makespan(const std::vector<int64_t>& unit_cost, int nC, int cus) {
+  const int64_t items = (int64_t)unit_cost.size() * nC;
+  const int64_t G = std::min<int64_t>(items, cus);
+  int64_t worst = 0;
+  for (int64_t b = 0; b < G; ++b) {
+    int64_t sum = 0;
+    for (int64_t i = b; i < items; i += G) sum += unit_cost[i / nC];
+    worst = std::max(worst, sum);
+  }
+  return worst;
+}
+
 // Work-unit lists of the B-stationary kernel.  The dense windows [0, W_tc)
 // are grouped by matrix (contiguous); list t splits each matrix's windows
-// into about 2^t * (its share of the chunks) contiguous, chunk-balanced units
-// (at least one per matrix).  Appends the lists to `units`.
-void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, int32_t W_tc,
+// into about kLdsTargets[t] * (its share of the chunks) contiguous,
+// chunk-balanced units (at least one per matrix), sorted by decreasing chunk
+// count.  Appends the lists to `units`.  pick[nC] is the list with the
+// smallest estimated makespan on `cus` compute units for nC column chunks.
+void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, int32_t W_tc, int cus,
                      std::vector<int32_t>& units, int32_t (&off)[stile::kLdsLists],
-                     int32_t (&cnt)[stile::kLdsLists], int32_t& max_k) {
+                     int32_t (&cnt)[stile::kLdsLists], int32_t& max_k, int8_t (&pick)[stile::kLdsPickMax + 1]) {
   // runs of windows of one matrix
   std::vector<int32_t> run_w0, run_m;
   for (int32_t w = 0; w < W_tc; ++w) {
@@ -343,7 +366,7 @@ void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, in
   max_k = 0;
   for (int t = 0; t < stile::kLdsLists; ++t) {
     off[t] = (int32_t)(units.size() / 4);
-    const int64_t target = (int64_t)1 << t;
+    const int64_t target = stile::kLdsTargets[t];
     for (size_t r = 0; r + 1 < run_w0.size(); ++r) {
       const int32_t w0 = run_w0[r], w1 = run_w0[r + 1], m = run_m[r];
       const int64_t c0 = tc_ptr[w0], chunks = tc_ptr[w1] - c0;
@@ -361,6 +384,39 @@ void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, in
       }
     }
     cnt[t] = (int32_t)(units.size() / 4) - off[t];
+    // largest units first: workgroups take units round-robin, so this order
+    // is close to longest-processing-time-first
+    std::vector<std::array<int32_t, 4>> lst(cnt[t]);
+    for (int32_t i = 0; i < cnt[t]; ++i)
+      for (int e = 0; e < 4; ++e) lst[i][e] = units[(size_t)(off[t] + i) * 4 + e];
+    std::stable_sort(lst.begin(), lst.end(), [&](const std::array<int32_t, 4>& x, const std::array<int32_t, 4>& y) {
+      return tc_ptr[x[1]] - tc_ptr[x[0]] > tc_ptr[y[1]] - tc_ptr[y[0]];
+    });
+    for (int32_t i = 0; i < cnt[t]; ++i)
+      for (int e = 0; e < 4; ++e) units[(size_t)(off[t] + i) * 4 + e] = lst[i][e];
+  }
+  // list choice per column-chunk count; lists with more than 4 items per CU
+  // are not considered once a smaller one fills the GPU (more restaging, and
+  // the estimate would cost more than it saves)
+  pick[0] = 0;
+  for (int nC = 1; nC <= stile::kLdsPickMax; ++nC) {
+    int best = 0;
+    int64_t best_cost = INT64_MAX;
+    for (int t = 0; t < stile::kLdsLists; ++t) {
+      const int64_t items = (int64_t)cnt[t] * nC;
+      if (t > 0 && items > 4 * (int64_t)cus && (int64_t)cnt[t - 1] * nC >= cus) break;
+      std::vector<int64_t> cost(cnt[t]);
+      for (int32_t i = 0; i < cnt[t]; ++i) {
+        const int32_t* u = &units[(size_t)(off[t] + i) * 4];
+        cost[i] = (int64_t)(tc_ptr[u[1]] - tc_ptr[u[0]]) + (u[3] + kLdsStageRows - 1) / kLdsStageRows;
+      }
+      const int64_t c = lds_makespan(cost, nC, cus);
+      if (c < best_cost) {
+        best_cost = c;
+        best = t;
+      }
+    }
+    pick[nC] = (int8_t)best;
   }
 }
 
@@ -497,7 +553,10 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   if (nchunks > INT32_MAX / 64) return fail(STILE_E_UNSUPPORTED, "too many dense chunks");
   std::vector<int32_t> lds_units;
   int32_t lds_off[stile::kLdsLists] = {}, lds_cnt[stile::kLdsLists] = {}, lds_max_k = 0;
-  if (lds_ok && W_tc > 0) build_lds_units(bd, tc_ptr, W_tc, lds_units, lds_off, lds_cnt, lds_max_k);
+  int8_t lds_pick[stile::kLdsPickMax + 1];
+  std::fill(std::begin(lds_pick), std::end(lds_pick), (int8_t)-1);
+  if (lds_ok && W_tc > 0)
+    build_lds_units(bd, tc_ptr, W_tc, stile::device_cus(), lds_units, lds_off, lds_cnt, lds_max_k, lds_pick);
   // Parts of the huge windows: consecutive ranges of at most kHugeChunks chunks.
   std::vector<int32_t> part_win, part_c0, part_c1;
   for (int32_t w = 0; w < W_huge; ++w)
@@ -635,6 +694,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
     d.lds_cnt[t] = lds_cnt[t];
   }
   d.lds_max_k = lds_max_k;
+  std::copy(std::begin(lds_pick), std::end(lds_pick), d.lds_pick);
   d.nchunks = (int32_t)nchunks;
   d.npieces = (int32_t)piece_row.size();
   d.piece_row = dptr<int32_t>(plan, o_prow);

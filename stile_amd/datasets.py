@@ -119,6 +119,22 @@ def pruned_layer(seed=0, M=3072, K=768, block=16, block_keep=0.10, residual=0.01
     return A, B
 
 
+def pruned_encoder(layers=24, d=1024, ffn=4096, seed=0):
+    """Block-pruned projections of a BERT-large-shaped encoder, as a batch of
+    uneven matrices: per layer Q, K, V, O (d x d) and the FFN input projection
+    (ffn x d), each like pruned_layer (10 % dense 16x16 blocks + 1 %
+    residual).  The FFN output projection (d x ffn) is left out: its K = ffn
+    is above the 1024 B rows the B-stationary kernel stages.  Returns the
+    matrices (CSR) and their (M, K) shapes."""
+    mats, shapes = [], []
+    for layer in range(layers):
+        for j, (m, k) in enumerate(((d, d), (d, d), (d, d), (d, d), (ffn, d))):
+            A, _ = pruned_layer(seed=seed + 1000 * layer + j, M=m, K=k, N=1)
+            mats.append(A)
+            shapes.append((m, k))
+    return mats, shapes
+
+
 def gcn_graph(scale=20, edge_factor=16, seed=0, feat=128):
     """Symmetric-normalised R-MAT adjacency D^-1/2 (A+I) D^-1/2, H and W."""
     n = 1 << scale

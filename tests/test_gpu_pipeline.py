@@ -211,3 +211,27 @@ def test_batched_bf16_matrices_on_the_b_stationary_kernel(cuda, N):
         assert rel <= 1e-2, f"matrix {m}x{k}: {rel:.3g}"
         r0, c0 = r0 + m, c0 + k
     assert np.array_equal(C, plan.run_spmm(_t(B, cuda)).cpu().numpy())
+
+
+@pytest.mark.parametrize("count,N", [(30, 1024), (12, 4160)])
+def test_batched_bf16_b_stationary_items_above_cus(cuda, count, N):
+    """More (unit, column chunk) items than compute units: each persistent
+    workgroup of the B-stationary kernel restages LDS for several items.  N =
+    1024 uses the plan's list choice for 16 column chunks; N = 4160 (65
+    chunks, the last one partial) the fallback choice.  Each matrix's rows of C
+    vs FP64 (1e-2), and run to run bit-identical."""
+    cycle = [(512, 256), (1024, 1024), (256, 768), (768, 512), (96, 1024), (1024, 128)]
+    shapes = [cycle[i % len(cycle)] for i in range(count)]
+    mats = [datasets.pruned_layer(seed=40 + i, M=m, K=k, N=8)[0] for i, (m, k) in enumerate(shapes)]
+    print(f"batched bf16 B-stationary persistent N={N} mats={count}", flush=True)
+    plan = api.build_spmm_plan_batched(mats, blk_dtype="bfloat16", N=N)
+    Ktot = sum(k for _, k in shapes)
+    B = np.random.default_rng(6).standard_normal((Ktot, N), dtype=np.float32)
+    Bd = _t(B, cuda)
+    C = plan.run_spmm(Bd).cpu().numpy()
+    r0 = c0 = 0
+    for A, (m, k) in zip(mats, shapes):
+        rel = ops.max_rel_err(C[r0:r0 + m], ops.spmm_fp64(A, B[c0:c0 + k]))
+        assert rel <= 1e-2, f"matrix {m}x{k}: {rel:.3g}"
+        r0, c0 = r0 + m, c0 + k
+    assert np.array_equal(C, plan.run_spmm(Bd).cpu().numpy())
