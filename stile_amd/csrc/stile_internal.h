@@ -36,6 +36,16 @@ constexpr int kLdsMaxK = 1024;
 constexpr int kLdsLists = 12;
 constexpr int kLdsTargets[kLdsLists] = {1, 2, 4, 8, 16, 32, 64, 128, 192, 256, 384, 512};
 constexpr int kLdsPickMax = 64;
+// Persistent launches with nC <= kLdsXcdMaxNC column chunks group items by
+// XCD: the GPU dispatches workgroup b to XCD b % kXcds, and XCD x takes the
+// contiguous item range [start[x], start[x+1]) (unit-major, chunk-minor, the
+// ranges balanced by estimated cost), so that a unit's column chunks and a
+// matrix's units run on one XCD together and share its L2.
+constexpr int kXcds = 8;
+constexpr int kLdsXcdMaxNC = 4;
+struct LdsXcdRanges {
+  int32_t start[kXcds + 1];
+};
 
 // Device view of a hybrid SpMM plan (window-major execution layout).
 //
@@ -86,6 +96,10 @@ struct SpmmPlanDev {
   int32_t lds_off[kLdsLists], lds_cnt[kLdsLists];
   int32_t lds_max_k;         // largest B-row count of a unit
   int8_t lds_pick[kLdsPickMax + 1];  // list to run for nC column chunks
+  // Host memory (read by the launcher only; owned by the plan):
+  // [kLdsLists][kLdsXcdMaxNC + 1][kXcds + 1] item ranges per XCD for list t
+  // and nC column chunks, or nullptr.
+  const int32_t* lds_xcd_host;
 };
 
 // Device view of a hybrid SDDMM plan.

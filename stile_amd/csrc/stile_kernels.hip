@@ -919,25 +919,42 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   store_window<true>(C, ldc, N, col, vec, sr, acc);
 }
 
-// units: the work-unit list of this launch, largest units first.  kLoop =
-// false: one workgroup per item (grid = items <= compute units); true:
-// persistent workgroups (grid = compute units) take items i = blockIdx.x,
-// blockIdx.x + gridDim.x, ... and restage the LDS block for each.  Two
-// instantiations because the loop costs the one-item launches about 0.6 us
-// (more registers live across the item body).
+// units: the work-unit list of this launch.  kLoop = false: one workgroup
+// per item (grid = items <= compute units); true: persistent workgroups that
+// restage the LDS block for each item they take: items blockIdx.x,
+// blockIdx.x + gridDim.x, ... or, with xcd (grid a multiple of kXcds), items
+// xr.start[x] + blockIdx.x / kXcds, + gridDim.x / kXcds, ... below
+// xr.start[x + 1] for x = blockIdx.x % kXcds.  Two instantiations because
+// the loop costs the one-item launches about 0.6 us (more registers live
+// across the item body).
 template <bool kLoop>
 __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p,
                                                                         const int32_t* __restrict__ tc_ptr,
                                                                         const int4* __restrict__ units,
                                                                         const float* __restrict__ B,
                                                                         int64_t ldb, int N, float* __restrict__ C,
-                                                                        int64_t ldc, int nC, int n_items, bool vec) {
+                                                                        int64_t ldc, int nC, int n_items, bool vec,
+                                                                        LdsXcdRanges xr, bool xcd) {
   extern __shared__ uint2 bsh_all[];  // [ksz][16]: 4 BF16 per entry
   if (!kLoop) {
     lds_item(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, blockIdx.x, bsh_all);
     return;
   }
-  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+  int beg = blockIdx.x, end = n_items, step = gridDim.x;
+  if (xcd) {
+    const int x = blockIdx.x % kXcds;
+    int r0 = 0, r1 = 0;
+#pragma unroll
+    for (int i = 0; i < kXcds; ++i)  // selects, not a dynamically indexed copy
+      if (i == x) {
+        r0 = xr.start[i];
+        r1 = xr.start[i + 1];
+      }
+    beg = r0 + (int)(blockIdx.x / kXcds);
+    end = r1;
+    step = gridDim.x / kXcds;
+  }
+  for (int item = beg; item < end; item += step) {
     __syncthreads();  // every wave is done reading the previous item's block
     lds_item(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, item, bsh_all);
   }
@@ -1030,6 +1047,16 @@ static int spmm_kernel_choice() {
   return choice;
 }
 
+// STILE_LDS_XCD=0 turns the XCD grouping of persistent B-stationary launches
+// off (for comparisons); unset or 1: on.  Read once.
+static bool lds_xcd_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("STILE_LDS_XCD");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 // STILE_LDS_LIST=t forces unit list t of the B-stationary kernel (for launch
 // shape sweeps); unset or out of range: the plan's choice for the column
 // chunk count.  Read once.
@@ -1068,13 +1095,25 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
     const int64_t items = (int64_t)p.lds_cnt[t] * nC;
     if (items > INT32_MAX) return hipErrorInvalidValue;
     // one workgroup per CU at most (each holds a staged block of LDS)
-    const int grid = (int)std::min<int64_t>(items, cus);
+    int grid = (int)std::min<int64_t>(items, cus);
+    LdsXcdRanges xr{};
+    const bool xcd = items > cus && nC <= kLdsXcdMaxNC && cus >= kXcds && p.lds_xcd_host && lds_xcd_enabled();
+    if (xcd) {
+      const int32_t* st = p.lds_xcd_host + ((size_t)t * (kLdsXcdMaxNC + 1) + nC) * (kXcds + 1);
+      if (st[0] != 0 || st[kXcds] != items) return hipErrorInvalidValue;  // ranges cover every item once
+      for (int x = 0; x < kXcds; ++x)
+        if (st[x + 1] < st[x]) return hipErrorInvalidValue;
+      std::copy(st, st + kXcds + 1, xr.start);
+      grid = cus / kXcds * kXcds;
+    }
     if (items <= cus)
       hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel<false>, dim3(grid), dim3(kLdsThreads), (size_t)p.lds_max_k * 128,
-                         s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C, ldc, nC, (int)items, vec);
+                         s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C, ldc, nC, (int)items, vec, xr,
+                         false);
     else
       hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel<true>, dim3(grid), dim3(kLdsThreads), (size_t)p.lds_max_k * 128,
-                         s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C, ldc, nC, (int)items, vec);
+                         s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C, ldc, nC, (int)items, vec, xr,
+                         xcd);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   } else if (p.W_tc > 0) {

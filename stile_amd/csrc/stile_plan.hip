@@ -39,6 +39,8 @@ struct StilePlan {
   int64_t n_tc_slots = 0, n_res = 0;
   const int32_t* tc_ids = nullptr;
   const int32_t* res_ids = nullptr;
+  // per-XCD item ranges of the B-stationary kernel (spmm.lds_xcd_host)
+  std::vector<int32_t> lds_xcd;
 };
 
 namespace {
@@ -328,9 +330,8 @@ int32_t window_matrix(const SpmmBuild& bd, int32_t w) {
 // which costs about one chunk per kLdsStageRows rows.
 constexpr int64_t kLdsStageRows = 8;
 
-// Longest workgroup of a launch of `items` items (unit i / nC, sorted by
-// decreasing size) over min(items, cus) persistent workgroups that take
-// items round-robin.
+// Longest workgroup of a launch of `items` items (unit i / nC) over
+// min(items, cus) persistent workgroups that take items round-robin.
 int64_t lds_makespan(const std::vector<int64_t>& unit_cost, int nC, int cus) {
   const int64_t items = (int64_t)unit_cost.size() * nC;
   const int64_t G = std::min<int64_t>(items, cus);
@@ -343,15 +344,53 @@ int64_t lds_makespan(const std::vector<int64_t>& unit_cost, int nC, int cus) {
   return worst;
 }
 
+// Contiguous item ranges per XCD (items unit-major: item i is unit i / nC),
+// cut where the running cost is nearest to x / kXcds of the total.
+void lds_xcd_split(const std::vector<int64_t>& unit_cost, int nC, int32_t (&start)[stile::kXcds + 1]) {
+  const int64_t items = (int64_t)unit_cost.size() * nC;
+  int64_t total = 0;
+  for (int64_t c : unit_cost) total += c * nC;
+  start[0] = 0;
+  int64_t i = 0, run = 0;  // run = cost of items [0, i)
+  for (int x = 1; x < stile::kXcds; ++x) {
+    const int64_t goal = total * x / stile::kXcds;
+    while (i < items && run + unit_cost[i / nC] <= goal) run += unit_cost[i++ / nC];
+    // item i straddles the goal: take it when that lands nearer
+    if (i < items && run + unit_cost[i / nC] - goal < goal - run) run += unit_cost[i++ / nC];
+    start[x] = (int32_t)i;
+  }
+  start[stile::kXcds] = (int32_t)items;
+}
+
+// Longest workgroup under the XCD grouping: XCD x's cus / kXcds workgroups
+// take its range round-robin.
+int64_t lds_xcd_makespan(const std::vector<int64_t>& unit_cost, int nC, int cus,
+                         const int32_t (&start)[stile::kXcds + 1]) {
+  const int per = std::max(1, cus / stile::kXcds);
+  int64_t worst = 0;
+  for (int x = 0; x < stile::kXcds; ++x)
+    for (int b = 0; b < per; ++b) {
+      int64_t sum = 0;
+      for (int64_t i = start[x] + b; i < start[x + 1]; i += per) sum += unit_cost[i / nC];
+      worst = std::max(worst, sum);
+    }
+  return worst;
+}
+
 // Work-unit lists of the B-stationary kernel.  The dense windows [0, W_tc)
 // are grouped by matrix (contiguous); list t splits each matrix's windows
 // into about kLdsTargets[t] * (its share of the chunks) contiguous,
-// chunk-balanced units (at least one per matrix), sorted by decreasing chunk
-// count.  Appends the lists to `units`.  pick[nC] is the list with the
-// smallest estimated makespan on `cus` compute units for nC column chunks.
+// chunk-balanced units (at least one per matrix).  A matrix's units stay
+// adjacent; matrices are ordered by decreasing largest unit (close to
+// longest-processing-time-first for workgroups that take units in turn).
+// Appends the lists to `units`.  pick[nC] is the list with the smallest
+// estimated makespan on `cus` compute units for nC column chunks, under the
+// mapping the launcher uses (XCD-grouped for persistent launches with nC <=
+// kLdsXcdMaxNC); xcd receives those launches' ranges per list and nC.
 void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, int32_t W_tc, int cus,
                      std::vector<int32_t>& units, int32_t (&off)[stile::kLdsLists],
-                     int32_t (&cnt)[stile::kLdsLists], int32_t& max_k, int8_t (&pick)[stile::kLdsPickMax + 1]) {
+                     int32_t (&cnt)[stile::kLdsLists], int32_t& max_k, int8_t (&pick)[stile::kLdsPickMax + 1],
+                     std::vector<int32_t>& xcd) {
   // runs of windows of one matrix
   std::vector<int32_t> run_w0, run_m;
   for (int32_t w = 0; w < W_tc; ++w) {
@@ -367,6 +406,8 @@ void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, in
   for (int t = 0; t < stile::kLdsLists; ++t) {
     off[t] = (int32_t)(units.size() / 4);
     const int64_t target = stile::kLdsTargets[t];
+    std::vector<std::vector<std::array<int32_t, 4>>> per_run(run_m.size());
+    std::vector<int64_t> run_key(run_m.size(), 0);  // largest unit of the run (chunks)
     for (size_t r = 0; r + 1 < run_w0.size(); ++r) {
       const int32_t w0 = run_w0[r], w1 = run_w0[r + 1], m = run_m[r];
       const int64_t c0 = tc_ptr[w0], chunks = tc_ptr[w1] - c0;
@@ -379,25 +420,23 @@ void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, in
         const int64_t goal = c0 + (chunks * u + n - 1) / n;
         int32_t b = u == n ? w1 : a + 1;
         while (b < w1 && tc_ptr[b] < goal) ++b;
-        units.insert(units.end(), {a, b, koff, ksz});
+        per_run[r].push_back({a, b, koff, ksz});
+        run_key[r] = std::max<int64_t>(run_key[r], tc_ptr[b] - tc_ptr[a]);
         a = b;
       }
     }
+    std::vector<size_t> order(run_m.size());
+    for (size_t r = 0; r < order.size(); ++r) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return run_key[x] > run_key[y]; });
+    for (size_t r : order)
+      for (const auto& u : per_run[r]) units.insert(units.end(), u.begin(), u.end());
     cnt[t] = (int32_t)(units.size() / 4) - off[t];
-    // largest units first: workgroups take units round-robin, so this order
-    // is close to longest-processing-time-first
-    std::vector<std::array<int32_t, 4>> lst(cnt[t]);
-    for (int32_t i = 0; i < cnt[t]; ++i)
-      for (int e = 0; e < 4; ++e) lst[i][e] = units[(size_t)(off[t] + i) * 4 + e];
-    std::stable_sort(lst.begin(), lst.end(), [&](const std::array<int32_t, 4>& x, const std::array<int32_t, 4>& y) {
-      return tc_ptr[x[1]] - tc_ptr[x[0]] > tc_ptr[y[1]] - tc_ptr[y[0]];
-    });
-    for (int32_t i = 0; i < cnt[t]; ++i)
-      for (int e = 0; e < 4; ++e) units[(size_t)(off[t] + i) * 4 + e] = lst[i][e];
   }
   // list choice per column-chunk count; lists with more than 4 items per CU
   // are not considered once a smaller one fills the GPU (more restaging, and
   // the estimate would cost more than it saves)
+  const int S = stile::kXcds + 1;
+  xcd.assign((size_t)stile::kLdsLists * (stile::kLdsXcdMaxNC + 1) * S, 0);
   pick[0] = 0;
   for (int nC = 1; nC <= stile::kLdsPickMax; ++nC) {
     int best = 0;
@@ -410,7 +449,15 @@ void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, in
         const int32_t* u = &units[(size_t)(off[t] + i) * 4];
         cost[i] = (int64_t)(tc_ptr[u[1]] - tc_ptr[u[0]]) + (u[3] + kLdsStageRows - 1) / kLdsStageRows;
       }
-      const int64_t c = lds_makespan(cost, nC, cus);
+      int64_t c;
+      if (nC <= stile::kLdsXcdMaxNC && items <= INT32_MAX) {
+        int32_t start[stile::kXcds + 1];
+        lds_xcd_split(cost, nC, start);
+        std::copy(start, start + S, xcd.begin() + ((size_t)t * (stile::kLdsXcdMaxNC + 1) + nC) * S);
+        c = items > cus ? lds_xcd_makespan(cost, nC, cus, start) : lds_makespan(cost, nC, cus);
+      } else {
+        c = lds_makespan(cost, nC, cus);
+      }
       if (c < best_cost) {
         best_cost = c;
         best = t;
@@ -418,6 +465,20 @@ void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, in
     }
     pick[nC] = (int8_t)best;
   }
+  // ranges of the lists the pick loop skipped (forced lists)
+  for (int t = 0; t < stile::kLdsLists; ++t)
+    for (int nC = 1; nC <= stile::kLdsXcdMaxNC; ++nC) {
+      int32_t* st = &xcd[((size_t)t * (stile::kLdsXcdMaxNC + 1) + nC) * S];
+      if (st[S - 1] != 0 || cnt[t] == 0) continue;
+      std::vector<int64_t> cost(cnt[t]);
+      for (int32_t i = 0; i < cnt[t]; ++i) {
+        const int32_t* u = &units[(size_t)(off[t] + i) * 4];
+        cost[i] = (int64_t)(tc_ptr[u[1]] - tc_ptr[u[0]]) + (u[3] + kLdsStageRows - 1) / kLdsStageRows;
+      }
+      int32_t start[stile::kXcds + 1];
+      lds_xcd_split(cost, nC, start);
+      std::copy(start, start + S, st);
+    }
 }
 
 int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
@@ -555,8 +616,10 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   int32_t lds_off[stile::kLdsLists] = {}, lds_cnt[stile::kLdsLists] = {}, lds_max_k = 0;
   int8_t lds_pick[stile::kLdsPickMax + 1];
   std::fill(std::begin(lds_pick), std::end(lds_pick), (int8_t)-1);
+  std::vector<int32_t> lds_xcd;
   if (lds_ok && W_tc > 0)
-    build_lds_units(bd, tc_ptr, W_tc, stile::device_cus(), lds_units, lds_off, lds_cnt, lds_max_k, lds_pick);
+    build_lds_units(bd, tc_ptr, W_tc, stile::device_cus(), lds_units, lds_off, lds_cnt, lds_max_k, lds_pick,
+                    lds_xcd);
   // Parts of the huge windows: consecutive ranges of at most kHugeChunks chunks.
   std::vector<int32_t> part_win, part_c0, part_c1;
   for (int32_t w = 0; w < W_huge; ++w)
@@ -695,6 +758,8 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   }
   d.lds_max_k = lds_max_k;
   std::copy(std::begin(lds_pick), std::end(lds_pick), d.lds_pick);
+  plan->lds_xcd = std::move(lds_xcd);
+  d.lds_xcd_host = plan->lds_xcd.empty() ? nullptr : plan->lds_xcd.data();
   d.nchunks = (int32_t)nchunks;
   d.npieces = (int32_t)piece_row.size();
   d.piece_row = dptr<int32_t>(plan, o_prow);
