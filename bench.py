@@ -348,7 +348,7 @@ def run_pruned_batch(args, dev):
                             f"residual; nnz={nnz}) x B 1024x128 each, one block-diagonal launch",
                 "matrices": len(mats), "M_total": Mt, "K_total": Kt, "N": N, "nnz": int(nnz),
                 "parallelism": "single GPU"},
-               roofline=roofline(alg, ms),
+               roofline=roofline(alg, ms, measured_traffic("hyb_spmm_traffic_pruned_batch.json")),
                plain_csr_gflops=_gf(flops, ms_csr), vendor_csr_gflops=_gf(flops, ms_vendor),
                hybrid_over_csr=round(ms_csr / ms, 3),
                hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),

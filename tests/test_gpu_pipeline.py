@@ -235,3 +235,42 @@ def test_batched_bf16_b_stationary_items_above_cus(cuda, count, N):
         assert rel <= 1e-2, f"matrix {m}x{k}: {rel:.3g}"
         r0, c0 = r0 + m, c0 + k
     assert np.array_equal(C, plan.run_spmm(Bd).cpu().numpy())
+
+
+def test_batched_bf16_xcd_grouped_launch_in_graph(cuda):
+    """120 small uneven matrices at N = 256 (4 column chunks): one unit per
+    matrix with dense windows already gives more (unit, chunk) items than the
+    GPU has CUs, so the B-stationary kernel runs persistent with its per-XCD
+    item ranges.  Eager result vs FP64 per matrix (1e-2); the same
+    launch captured in a HIP graph and replayed three times, with eager work
+    and a synchronisation between replays, equals the eager result bit for
+    bit each time."""
+    import torch
+    rng = np.random.default_rng(12)
+    shapes = [(int(16 * rng.integers(2, 33)), int(16 * rng.integers(4, 65))) for _ in range(120)]
+    mats = [datasets.pruned_layer(seed=100 + i, M=m, K=k, N=8)[0] for i, (m, k) in enumerate(shapes)]
+    N = 256
+    print(f"batched bf16 XCD-grouped N={N} mats={len(mats)} + graph", flush=True)
+    plan = api.build_spmm_plan_batched(mats, blk_dtype="bfloat16", N=N)
+    Ktot = sum(k for _, k in shapes)
+    Mtot = sum(m for m, _ in shapes)
+    B = rng.standard_normal((Ktot, N), dtype=np.float32)
+    Bd = _t(B, cuda)
+    C_eager = plan.run_spmm(Bd).cpu().numpy()
+    r0 = c0 = 0
+    for A, (m, k) in zip(mats, shapes):
+        rel = ops.max_rel_err(C_eager[r0:r0 + m], ops.spmm_fp64(A, B[c0:c0 + k]))
+        assert rel <= 1e-2, f"matrix {m}x{k}: {rel:.3g}"
+        r0, c0 = r0 + m, c0 + k
+    C = torch.empty((Mtot, N), device=cuda)
+    plan.run_spmm(Bd, C)  # warm up outside the capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.run_spmm(Bd, C)
+    for r in range(3):
+        C.fill_(float("nan"))  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(C.cpu().numpy(), C_eager), f"replay {r}"
