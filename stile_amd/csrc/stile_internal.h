@@ -42,6 +42,9 @@ constexpr int kLdsPickMax = 64;
 // ranges balanced by estimated cost), so that a unit's column chunks and a
 // matrix's units run on one XCD together and share its L2.
 constexpr int kXcds = 8;
+// Sparse-only rows of an LDS-kernel plan with at most this many (row, 64
+// columns) tasks run as the LDS launch's tail instead of a launch of their own.
+constexpr int kLdsTailMaxTasks = 1024;
 constexpr int kLdsXcdMaxNC = 4;
 struct LdsXcdRanges {
   int32_t start[kXcds + 1];

@@ -689,11 +689,11 @@ __global__ __launch_bounds__(256, 7) void hyb_spmm_sparse_kernel(SpmmPlanDev p, 
 // in flight per group; the four partial float4 are added by xor shuffles and
 // group 0 stores the row.  Rows with more than kHubNnz sparse nonzeros sum
 // with Neumaier compensation, as in finish_slot.
-__global__ __launch_bounds__(256) void hyb_spmm_sparse_split_kernel(SpmmPlanDev p, const float* __restrict__ B,
-                                                                    int64_t ldb, int N, float* __restrict__ C,
-                                                                    int64_t ldc, int nC, bool vec) {
+// One wave's task `wave` = (row wave / nC, column chunk wave % nC).
+__device__ __forceinline__ void sparse_split_wave(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
+                                                  int N, float* __restrict__ C, int64_t ldc, int nC, bool vec,
+                                                  int64_t wave) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int64_t rows = (int64_t)(p.W - p.W_tc) * 16;
   if (wave >= rows * nC) return;
   const int64_t ri = wave / nC;
@@ -747,6 +747,12 @@ __global__ __launch_bounds__(256) void hyb_spmm_sparse_split_kernel(SpmmPlanDev 
     s.w += __shfl_xor(s.w, off);
   }
   if (g == 0 && row >= 0) store_c4(C, ldc, row, col, N, vec, s);
+}
+
+__global__ __launch_bounds__(256) void hyb_spmm_sparse_split_kernel(SpmmPlanDev p, const float* __restrict__ B,
+                                                                    int64_t ldb, int N, float* __restrict__ C,
+                                                                    int64_t ldc, int nC, bool vec) {
+  sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
 }
 
 // B-stationary variant for BF16 plans whose matrices all have K <= kLdsMaxK:
@@ -927,6 +933,13 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
 // xr.start[x + 1] for x = blockIdx.x % kXcds.  Two instantiations because
 // the loop costs the one-item launches about 0.6 us (more registers live
 // across the item body).
+//
+// Tail: the last tail_blocks workgroups of the grid then run the sparse-only
+// rows' tasks (one wave per (row, 64 columns), as hyb_spmm_sparse_split_kernel)
+// in place of a second launch: block gridDim.x - 1 - i takes tasks 16i ..
+// 16i + 15 of tail_tasks.  Under the XCD grouping those are the last slot of
+// each XCD, which holds the fewest items when a range does not divide evenly;
+// one-item launches add tail blocks past the items when CUs are free.
 template <bool kLoop>
 __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p,
                                                                         const int32_t* __restrict__ tc_ptr,
@@ -934,10 +947,16 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
                                                                         const float* __restrict__ B,
                                                                         int64_t ldb, int N, float* __restrict__ C,
                                                                         int64_t ldc, int nC, int n_items, bool vec,
-                                                                        LdsXcdRanges xr, bool xcd) {
+                                                                        LdsXcdRanges xr, bool xcd, int tail_blocks,
+                                                                        int tail_tasks) {
   extern __shared__ uint2 bsh_all[];  // [ksz][16]: 4 BF16 per entry
+  const int tb = (int)gridDim.x - 1 - (int)blockIdx.x;
   if (!kLoop) {
-    lds_item(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, blockIdx.x, bsh_all);
+    if ((int)blockIdx.x < n_items) lds_item(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, blockIdx.x, bsh_all);
+    if (tb < tail_blocks) {
+      const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
+      if (task < tail_tasks) sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, task);
+    }
     return;
   }
   int beg = blockIdx.x, end = n_items, step = gridDim.x;
@@ -957,6 +976,10 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   for (int item = beg; item < end; item += step) {
     __syncthreads();  // every wave is done reading the previous item's block
     lds_item(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, item, bsh_all);
+  }
+  if (tb < tail_blocks) {
+    const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
+    if (task < tail_tasks) sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, task);
   }
 }
 
@@ -1082,10 +1105,16 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
   // kLdsPickMax), else the smallest list with at least one item per CU.
   bool lds = p.lds_units != nullptr && p.W_tc > 0;
   if (spmm_kernel_choice() == 1) lds = false;
+  // Sparse-only rows: with the LDS kernel and at most kLdsTailMaxTasks
+  // (row, 64 columns) tasks they are the LDS launch's tail; otherwise a
+  // launch of their own follows the dense kernels.
+  const int64_t so_rows = (int64_t)(p.W - p.W_tc) * 16;
+  const int cus = device_cus();
+  const bool fold = lds && so_rows > 0 && so_rows * nC <= kLdsTailMaxTasks &&
+                    (so_rows * nC + kLdsThreads / 64 - 1) / (kLdsThreads / 64) <= cus / kXcds * kXcds;
   // The LDS kernel runs every dense window; otherwise the streaming kernel
   // does (its first blocks the big windows).  The sparse-only windows follow.
   if (lds) {
-    const int cus = device_cus();
     int t = lds_list_override();
     if (t < 0 && nC <= kLdsPickMax) t = p.lds_pick[nC];
     if (t < 0) {
@@ -1106,14 +1135,20 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
       std::copy(st, st + kXcds + 1, xr.start);
       grid = cus / kXcds * kXcds;
     }
+    const int tail_tasks = fold ? (int)(so_rows * nC) : 0;
+    const int tail_blocks = (tail_tasks + kLdsThreads / 64 - 1) / (kLdsThreads / 64);
+    // one-item launches: tail blocks of their own past the items while CUs
+    // are free, else the last item blocks take the tail
+    if (items <= cus && items + tail_blocks <= cus) grid = (int)items + tail_blocks;
+    if (items <= cus) grid = std::max(grid, tail_blocks);
     if (items <= cus)
       hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel<false>, dim3(grid), dim3(kLdsThreads), (size_t)p.lds_max_k * 128,
                          s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C, ldc, nC, (int)items, vec, xr,
-                         false);
+                         false, tail_blocks, tail_tasks);
     else
       hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel<true>, dim3(grid), dim3(kLdsThreads), (size_t)p.lds_max_k * 128,
                          s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C, ldc, nC, (int)items, vec, xr,
-                         xcd);
+                         xcd, tail_blocks, tail_tasks);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   } else if (p.W_tc > 0) {
@@ -1141,11 +1176,12 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  // Sparse-only rows one per lane group, or, when that leaves fewer than 16
-  // waves per CU for rows of 8 or more nonzeros on average, one per wave.
-  const int64_t so_rows = (int64_t)(p.W - p.W_tc) * 16;
-  const bool split = so_rows > 0 && so_rows * nC / 4 < 4096 && p.nnz_sparse_only >= 8 * so_rows;
-  const int64_t sparse_waves = (int64_t)((p.n_mix + 3) / 4) * nC + (split ? 0 : so_rows * nC / 4);
+  // Sparse-only rows (unless folded) one per lane group, or, when that
+  // leaves fewer than 16 waves per CU for rows of 8 or more nonzeros on
+  // average, one per wave.
+  const int64_t so_left = fold ? 0 : so_rows;
+  const bool split = so_left > 0 && so_left * nC / 4 < 4096 && p.nnz_sparse_only >= 8 * so_left;
+  const int64_t sparse_waves = (int64_t)((p.n_mix + 3) / 4) * nC + (split ? 0 : so_left * nC / 4);
   if (split) {
     const int64_t blocks = (so_rows * nC + 3) / 4;
     hipLaunchKernelGGL(hyb_spmm_sparse_split_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, B, ldb, N, C,
@@ -1157,7 +1193,7 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
     const int64_t blocks = (sparse_waves + 3) / 4;
     if (blocks > INT32_MAX) return hipErrorInvalidValue;
     hipLaunchKernelGGL(hyb_spmm_sparse_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, B, ldb, N, C, ldc, nC,
-                       vec, split ? (int64_t)0 : so_rows * nC / 4);
+                       vec, split ? (int64_t)0 : so_left * nC / 4);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -252,3 +252,29 @@ def test_malformed_packs_are_rejected(cuda):
 
     for edit in (ell_col, tc_col, tc_indptr, ell_row):
         broken(edit)
+
+
+@pytest.mark.parametrize("N", [64, 128, 512])
+def test_bf16_lds_plan_with_sparse_only_rows(cuda, N):
+    """B-stationary plan whose lower half of rows has no dense block (sparse-only
+    windows, 7..31 nonzeros per row): 256 rows x nC tasks ride as the LDS
+    launch's tail at N = 64 and 128 (256 and 512 tasks) and get a launch of
+    their own at N = 512 (2048 tasks).  vs FP64 (1e-2), run to run bit-identical."""
+    from stile_amd import _native
+    rng = np.random.default_rng(21)
+    top, _ = datasets.pruned_layer(seed=21, M=256, K=512, N=8)
+    bottom = scipy.sparse.random(256, 512, density=0.04, format="csr", random_state=22, dtype=np.float32)
+    A = scipy.sparse.vstack([top, bottom], format="csr")
+    pack = _packs.spmm_pack(A, min_fill=64)
+    B = rng.standard_normal((512, N), dtype=np.float32)
+    print(f"bf16 LDS plan with sparse-only rows N={N}", flush=True)
+    plan = _native.Plan.spmm(pack, "bfloat16")
+    info = plan.info()
+    assert info["block_nnz"] + info["sparse_nnz"] == A.nnz
+    Bd = _t(B, cuda)
+    C = plan.run_spmm(Bd).cpu().numpy()
+    rel = ops.max_rel_err(C, ops.spmm_fp64(A, B))
+    assert rel <= 1e-2, f"relative error {rel:.3g} > 1e-2"
+    # the sparse-only rows are FP32 products
+    _check_fp32(C[256:], bottom, B)
+    assert np.array_equal(C, plan.run_spmm(Bd).cpu().numpy())
