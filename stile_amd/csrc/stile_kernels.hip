@@ -19,6 +19,7 @@
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 
@@ -1047,15 +1048,16 @@ void prepare_spmm_kernels(const SpmmPlanDev& p) {
 }
 
 int device_cus() {
-  static int cus[64] = {};
+  // per-device cache; threads that race on the first query store the same value
+  static std::atomic<int> cus[64] = {};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cus[dev] == 0) {
-    int n = 0;
+  int n = cus[dev].load(std::memory_order_relaxed);
+  if (n == 0) {
     if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-    cus[dev] = n;
+    cus[dev].store(n, std::memory_order_relaxed);
   }
-  return cus[dev];
+  return n;
 }
 
 // STILE_SPMM_KERNEL=stream forces the streaming dense kernel for BF16 plans
