@@ -840,25 +840,6 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   const int4 unit = units[item / nC];
   const int n0 = chunk * 64;
   const int koff = unit.z, ksz = unit.w;
-  // Stage B[koff:koff+ksz, n0:n0+64] as BF16: eight independent float4 loads
-  // per thread in flight before the packed stores.
-  for (int base = threadIdx.x; base < ksz * 16; base += 8 * kLdsThreads) {
-    float4 v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = base + u * kLdsThreads;
-      v[u] = i < ksz * 16 ? load_b4(B, ldb, koff + (i >> 4), n0 + 4 * (i & 15), N, vec)
-                          : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = base + u * kLdsThreads;
-      if (i < ksz * 16) bsh_all[i] = make_uint2(bf16_pair(v[u].x, v[u].y), bf16_pair(v[u].z, v[u].w));
-    }
-  }
-  __syncthreads();
-  // the unit's column ids k lie in [koff, koff + ksz): LDS row k - koff
-  const uint2* bsh = bsh_all - (int64_t)koff * 16;
   // The wave index is made visibly uniform so that the window and chunk
   // bookkeeping below lives in scalar registers and reads tc_ptr with scalar
   // loads: a vector load of the next window bound inside the loop would make
@@ -881,24 +862,51 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   };
   const int w_beg = window_at(cb + (int)(((int64_t)(ce - cb) * wib) / kWaves));
   const int w_end = window_at(cb + (int)(((int64_t)(ce - cb) * (wib + 1)) / kWaves));
-  if (w_beg >= w_end) return;
-  const int c_beg = tc_ptr[w_beg], c_end = tc_ptr[w_end];
+  const bool has = w_beg < w_end;  // wave-uniform
+  const int c_beg = has ? tc_ptr[w_beg] : 0, c_end = has ? tc_ptr[w_end] : 1;
 
   // The operands of chunk c + 2 are requested while chunk c is multiplied
   // (indices clamped to the wave's last chunk, so every load is
   // unconditional).  Three register sets rotate through a loop unrolled by
   // three, so no set is copied (a copy would wait for the newest load).
   // Every window of the run holds at least one chunk, so at most one window
-  // ends before each chunk.
+  // ends before each chunk.  The first window's rows and the first two
+  // chunks' operands are requested before the B block is staged, so their
+  // latency overlaps the staging.
   const int last = c_end - 1;
   int w = w_beg;
-  int w_next = tc_ptr[w + 1];
-  SlotRows sr = load_rows(p, w * 16 + g * 4);
+  int w_next = 0;
+  SlotRows sr{};
+  ChunkRegs r0{}, r1{};
+  if (has) {
+    w_next = tc_ptr[w + 1];
+    sr = load_rows(p, w * 16 + g * 4);
+    r0 = load_chunk_regs(p, c_beg, g, lane);
+    r1 = load_chunk_regs(p, min(c_beg + 1, last), g, lane);
+  }
+  // Stage B[koff:koff+ksz, n0:n0+64] as BF16: eight independent float4 loads
+  // per thread in flight before the packed stores.
+  for (int base = threadIdx.x; base < ksz * 16; base += 8 * kLdsThreads) {
+    float4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + u * kLdsThreads;
+      v[u] = i < ksz * 16 ? load_b4(B, ldb, koff + (i >> 4), n0 + 4 * (i & 15), N, vec)
+                          : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + u * kLdsThreads;
+      if (i < ksz * 16) bsh_all[i] = make_uint2(bf16_pair(v[u].x, v[u].y), bf16_pair(v[u].z, v[u].w));
+    }
+  }
+  __syncthreads();
+  if (!has) return;
+  // the unit's column ids k lie in [koff, koff + ksz): LDS row k - koff
+  const uint2* bsh = bsh_all - (int64_t)koff * 16;
   f32x4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  ChunkRegs r0 = load_chunk_regs(p, c_beg, g, lane);
-  ChunkRegs r1 = load_chunk_regs(p, min(c_beg + 1, last), g, lane);
   ChunkRegs r2;
   int c = c_beg;
   auto step = [&](const ChunkRegs& use, ChunkRegs& fill) {
