@@ -690,7 +690,13 @@ __global__ __launch_bounds__(256, 7) void hyb_spmm_sparse_kernel(SpmmPlanDev p, 
 // in flight per group; the four partial float4 are added by xor shuffles and
 // group 0 stores the row.  Rows with more than kHubNnz sparse nonzeros sum
 // with Neumaier compensation, as in finish_slot.
-// One wave's task `wave` = (row wave / nC, column chunk wave % nC).
+// One wave's task `wave` = (row wave / nC, column chunk wave % nC).  The
+// row's nonzeros go 64 at a time: lane (g, j) loads nonzero base + 16g + j
+// (column and value, one load each, so a round costs one index latency),
+// lane group g's sixteen are passed to its lanes by shuffles, and the
+// group's 16 B-row gathers are in flight together.  Positions past r1 read
+// the row's last entry with weight 0.  Fixed order: within a group u = 0..15
+// per round, then the xor-shuffle sum over the four groups.
 __device__ __forceinline__ void sparse_split_wave(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
                                                   int N, float* __restrict__ C, int64_t ldc, int nC, bool vec,
                                                   int64_t wave) {
@@ -707,35 +713,35 @@ __device__ __forceinline__ void sparse_split_wave(const SpmmPlanDev& p, const fl
   const int r1 = min(rend, r0 + kSplitNnz);
   const bool compensated = rend - r0 > kHubNnz;
   float4 s = make_float4(0.f, 0.f, 0.f, 0.f), cc = make_float4(0.f, 0.f, 0.f, 0.f);
-  // positions past r1 read the row's last entry with weight 0
-  for (int q0 = r0 + g; q0 < r1; q0 += 32) {
-    int k[8];
-    float v[8];
+  for (int base = r0; base < r1; base += 64) {
+    const int q = base + lane;  // = base + 16g + j
+    const int qc = min(q, r1 - 1);
+    const int kq = p.res_cols[qc];
+    const float vq = q < r1 ? p.res_vals[qc] : 0.f;
+    float4 b[16];
+    float v[16];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int q = q0 + 4 * i;
-      const int qc = min(q, r1 - 1);
-      k[i] = p.res_cols[qc];
-      v[i] = q < r1 ? p.res_vals[qc] : 0.f;
+    for (int u = 0; u < 16; ++u) {
+      const int src = (lane & 48) + u;
+      const int k = __shfl(kq, src);
+      v[u] = __shfl(vq, src);
+      b[u] = load_b4(B, ldb, k, col, N, vec);
     }
-    float4 b[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) b[i] = load_b4(B, ldb, k[i], col, N, vec);
     if (compensated) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        neu_add(s.x, cc.x, v[i] * b[i].x);
-        neu_add(s.y, cc.y, v[i] * b[i].y);
-        neu_add(s.z, cc.z, v[i] * b[i].z);
-        neu_add(s.w, cc.w, v[i] * b[i].w);
+      for (int u = 0; u < 16; ++u) {
+        neu_add(s.x, cc.x, v[u] * b[u].x);
+        neu_add(s.y, cc.y, v[u] * b[u].y);
+        neu_add(s.z, cc.z, v[u] * b[u].z);
+        neu_add(s.w, cc.w, v[u] * b[u].w);
       }
     } else {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        s.x += v[i] * b[i].x;
-        s.y += v[i] * b[i].y;
-        s.z += v[i] * b[i].z;
-        s.w += v[i] * b[i].w;
+      for (int u = 0; u < 16; ++u) {
+        s.x += v[u] * b[u].x;
+        s.y += v[u] * b[u].y;
+        s.z += v[u] * b[u].z;
+        s.w += v[u] * b[u].w;
       }
     }
   }
