@@ -92,7 +92,8 @@ struct SpmmPlanDev {
   const uint32_t* tc_mask;   // [nchunks][16] entry bits (attention plans), else nullptr
   // B-stationary work units (BF16 plans with every matrix's K <= kLdsMaxK):
   // unit = (first window, end window, first B row, B rows) of one matrix,
-  // windows in [0, W_tc).  List t (about 2^t units, chunk-balanced) starts at
+  // windows in [0, W_tc).  List t (about kLdsTargets[t] units, chunk-balanced,
+  // a matrix's units adjacent) starts at
   // lds_units[lds_off[t]] and holds lds_cnt[t] units; lds_units is nullptr
   // when the plan does not qualify.
   const int4* lds_units;
