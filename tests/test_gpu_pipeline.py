@@ -274,3 +274,47 @@ def test_batched_bf16_xcd_grouped_launch_in_graph(cuda):
         g.replay()
         torch.cuda.synchronize()
         assert np.array_equal(C.cpu().numpy(), C_eager), f"replay {r}"
+
+
+@pytest.mark.parametrize("count,N,every", [(40, 1024, 40), (120, 256, 12)])
+def test_batched_bf16_persistent_with_sparse_only_tail(cuda, count, N, every):
+    """Persistent B-stationary launches (items above the CUs: round-robin at
+    N = 1024, per-XCD ranges at N = 256) whose plan also has sparse-only rows:
+    every `every`-th matrix gets 16 rows without a dense block (with the
+    windows that hold none by chance: 3 and 14 sparse-only windows, 768 and
+    896 (row, 64 columns) tasks, within the tail limit of 1,024), and the
+    rows' tasks run as the launch's tail.  Each matrix's rows vs FP64
+    (1e-2), the appended rows within the FP32 bound, run to run
+    bit-identical."""
+    rng = np.random.default_rng(30)
+    mats, shapes, tails = [], [], []
+    for i in range(count):
+        m, k = [(256, 1024), (128, 1024), (384, 768), (64, 1024)][i % 4]
+        A, _ = datasets.pruned_layer(seed=200 + i, M=m, K=k, N=8)
+        if i % every == 0:
+            S = scipy.sparse.random(16, k, density=0.03, format="csr", random_state=300 + i, dtype=np.float32)
+            A = scipy.sparse.vstack([A, S], format="csr")
+            tails.append((len(mats), S))
+        mats.append(A)
+        shapes.append(A.shape)
+    print(f"batched bf16 persistent + sparse-only tail N={N} mats={count}", flush=True)
+    # packed directly (dense 16x16 blocks as TC tiles, the rest ELL), so the
+    # appended rows are sparse-only whatever a search would pick
+    from stile_amd import _native
+    from tests import _packs
+    plan = _native.Plan.spmm([_packs.spmm_pack(A, min_fill=64) for A in mats], "bfloat16")
+    Ktot = sum(k for _, k in shapes)
+    B = rng.standard_normal((Ktot, N), dtype=np.float32)
+    Bd = _t(B, cuda)
+    C = plan.run_spmm(Bd).cpu().numpy()
+    offs = np.cumsum([(0, 0)] + shapes, axis=0)
+    for i, A in enumerate(mats):
+        (r0, c0), (m, k) = offs[i], shapes[i]
+        rel = ops.max_rel_err(C[r0:r0 + m], ops.spmm_fp64(A, B[c0:c0 + k]))
+        assert rel <= 1e-2, f"matrix {i} {m}x{k}: {rel:.3g}"
+    for i, S in tails:
+        (r0, c0), (m, k) = offs[i], shapes[i]
+        ref = ops.spmm_fp64(S, B[c0:c0 + k])
+        bound = ops.spmm_abs_bound(S, B[c0:c0 + k])
+        assert ops.bound_ratio(C[r0 + m - 16:r0 + m], ref, bound, int(np.diff(S.indptr).max(initial=1))) <= 1.0
+    assert np.array_equal(C, plan.run_spmm(Bd).cpu().numpy())
