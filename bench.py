@@ -538,7 +538,8 @@ def run_attention(args, dev):
     # the unfused pipeline (SDDMM -> row softmax -> value refresh -> SpMM, one
     # graph) with the default search, for comparison
     progress("attention: unfused pipeline for comparison")
-    un = BatchedSparseAttention(masks, d, blk_dtype="bfloat16", device=dev, fused=False)
+    un = BatchedSparseAttention(masks, d, blk_dtype="bfloat16", device=dev, fused=False,
+                                cache_dir=cache_dir(args, "attention"))
     un.capture(Q, K, V)
     ms_unfused = time_launches(un.replay, max(args.steps // 10, 5), 2)
     unfused_equal = float((un.gout - att.gout).abs().max())

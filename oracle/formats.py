@@ -122,6 +122,90 @@ def get_valid_tile_pos_list_for_ELL(hyb_getnnz_minus, hyb_short_begin_row, n_pie
     return [(i, 0, 0) for i in list1]
 
 
+def _factors(val):
+    """get_factors, gen_formats_v2.py (divisors of val, ascending)."""
+    return [i for i in range(1, val + 1) if val % i == 0]
+
+
+def _tiles(val, split_n, base):
+    """get_tiles, gen_formats_v2.py: every split of val into split_n - 1 factors
+    appended to base."""
+    if split_n == 2:
+        return [base + (i,) for i in _factors(val)]
+    out = []
+    for i in _factors(val):
+        out += _tiles(val // i, split_n - 1, base + (i,))
+    return out
+
+
+def ell_tb_tile_shapes(n_pieces, max_bucket_size):
+    """Thread-block ELL tile shapes the search enumerates for an ELL op of
+    n_pieces split rows, transcribed:
+      get_tile_sizes_idx 'p' branch with is_tb_tile (gen_formats_v2.py:1947-1956):
+        I = (None, 2^t) for t <= log2(n_pieces), K = (1, 2^t) for t <= log2(mbs);
+      J fixed to (None, 32) by search_best_tile_given_op_BSPMM
+        (my_branch_and_bound.py:751-758);
+      itertools.product in (I, J, K) order, then tile_size_heuristics with
+        is_tb_tile (gen_formats_v2.py:1188-1214): I*J a multiple of 128,
+        I*J >= 256 and I*K == 8*mbs.
+    Pinned: test_ell_tile_shapes_known_answer."""
+    import itertools
+    idx0 = [(None, 2 ** t) for t in range(int(math.log(n_pieces, 2)) + 1)]
+    idx1 = [(None, 32)]
+    idx2 = [(1, 2 ** t) for t in range(int(math.log(max_bucket_size, 2)) + 1)]
+    out = []
+    for ts in itertools.product(idx0, idx1, idx2):
+        threads = ts[0][1] * ts[1][1]
+        if threads % 128 != 0 or threads < 256:
+            continue
+        if ts[0][1] * ts[2][1] != max_bucket_size * 8:
+            continue
+        out.append(ts)
+    return out
+
+
+def tc_tile_sizes(op_type='spmm', k_notsorted=False):
+    """get_TC_tile_sizes, gen_formats_v2.py:1252-1281 (no TC_tile_sizes
+    override): SpMM (16, 32, 16*i) for i = 10..1 (k_notsorted: i = 1 only);
+    SDDMM (16, 16*2^j, 16) for j = 0..2.  Pinned: test_tc_tile_sizes_known_answer."""
+    if k_notsorted:
+        return [((None, 16), (None, 32), (None, 16 * i)) for i in range(1, 2)]
+    if op_type == 'sddmm':
+        return [((None, 16), (None, 16 * (2 ** j)), (None, 16 * i)) for i in range(1, 2) for j in range(3)]
+    return [((None, 16), (None, 32), (None, 16 * i)) for i in range(160 // 16, 0, -1)]
+
+
+def sddmm_tc_tile_sizes(k_feat):
+    """get_tile_sizes_idx, 'TC_sddmm' branch (gen_formats_v2.py:1966-1971):
+    the SDDMM TC shapes whose j size is the largest one <= K_feat."""
+    idx = tc_tile_sizes('sddmm')
+    tile_j = max(j for j in set(ts[1][1] for ts in idx) if j <= k_feat)
+    return [ts for ts in idx if ts[1][1] == tile_j]
+
+
+def tile_params(template, tile_sizes, max_bucket_size, is_tb_tile=True):
+    """get_params_list, gen_formats_v2.py:2036-2068, per template string
+    ('sparse_template_ell', 'TensorCore_template', '1D_sddmm', 'TC_sddmm').
+    Pinned: test_tile_params_known_answer."""
+    out = []
+    if template in ('sparse_template_ell', 'TensorCore_template'):
+        for unroll in ([True] if is_tb_tile else [True, False]):
+            params = {'use_implicit_unroll': unroll}
+            if template == 'sparse_template_ell':
+                params['last_bucket'] = math.prod(tile_sizes[2][1:]) == max_bucket_size
+            else:
+                if not unroll:
+                    continue
+                params['mma_shape_str'] = "m8n32k16" if tile_sizes == ((None, 8), (None, 32), (None, 16)) \
+                    else "m16n16k16"
+            out.append(params)
+    elif template == '1D_sddmm':
+        out.append({'tx': 8, 'ty': 4, 'vec_size': 4, 'group_size': 8, 'max_bucket_size': max_bucket_size})
+    elif template == 'TC_sddmm':
+        out.append({'mma_shape_str': "m16n16k16", 'warp_num': 1, 'vec_size': 4})
+    return out
+
+
 def estimate_max_bucket_size(nnz, SM_num=108, tot_thread_per_SM=2048):
     """my_search_formats.py:965-981, transcribed."""
     thread_num = 256

@@ -111,6 +111,21 @@ _SIGNATURES = {
 }
 
 _lib = None
+# set once this process has made a HIP call through the library (plan
+# creation starts the HIP runtime); api.parallel_map then spawns its workers
+# instead of forking
+_hip_started = False
+
+
+def hip_started():
+    """True once this process has created a device plan (the HIP runtime is
+    initialised, so forked children could not use it)."""
+    return _hip_started
+
+
+def _mark_hip():
+    global _hip_started
+    _hip_started = True
 
 
 class StileError(RuntimeError):
@@ -359,6 +374,7 @@ class Plan:
         arrays = _PackArrays()
         structs = (StileSpmmPack * len(packs))(*[_spmm_pack_struct(p, arrays) for p in packs])
         h = ctypes.c_void_p()
+        _mark_hip()
         _check(lib.stile_plan_create_spmm_batched(structs, len(packs), blk_dtype_code(blk_dtype),
                                                   ctypes.byref(h)))
         return cls(h, "spmm", sum(p["M"] for p in packs), sum(p["K"] for p in packs))
@@ -372,6 +388,7 @@ class Plan:
         arrays = _PackArrays()
         structs = (StileSpmmPack * len(packs))(*[_spmm_pack_struct(p, arrays) for p in packs])
         h = ctypes.c_void_p()
+        _mark_hip()
         _check(lib.stile_plan_create_attention(structs, len(packs), ctypes.byref(h)))
         plan = cls(h, "spmm", sum(p["M"] for p in packs), sum(p["K"] for p in packs))
         plan.attention_plan = True
@@ -401,6 +418,7 @@ class Plan:
         arrays = _PackArrays()
         structs = (StileSddmmPack * len(packs))(*[_sddmm_pack_struct(p, arrays) for p in packs])
         h = ctypes.c_void_p()
+        _mark_hip()
         _check(lib.stile_plan_create_sddmm_batched(structs, len(packs), blk_dtype_code(blk_dtype),
                                                    ctypes.byref(h)))
         return cls(h, "sddmm", sum(p["M"] for p in packs), sum(p["N"] for p in packs),

@@ -114,32 +114,45 @@ def sddmm_pack(A, opts=None, Kf=32):
 
 def parallel_map(fn, jobs, workers=None, log=None):
     """fn over jobs in worker processes: forked while this process has not
-    touched the GPU, spawned otherwise (a forked copy of an initialised HIP
-    runtime is not usable).  Spawned workers re-import the main module, so a
-    script calling this after GPU work needs the `if __name__ == "__main__"`
-    guard."""
+    started the HIP runtime, spawned otherwise (a forked copy of an
+    initialised HIP runtime is not usable).  The runtime counts as started once
+    torch has initialised CUDA or this package has created a device plan.
+    Spawned workers re-import the main module, so a script calling this after
+    GPU work needs the `if __name__ == "__main__"` guard.  The pool is closed
+    and joined (not terminated) when every job is done."""
     import multiprocessing as mp
     import os
     workers = min(len(jobs), workers or min(os.cpu_count() or 1, 16))
     if workers <= 1:
         return [fn(j) for j in jobs]
+    from . import _native
+    spawn = _native.hip_started()
     try:
         import torch
-        spawn = torch.cuda.is_initialized()
+        spawn = spawn or torch.cuda.is_initialized()
     except ImportError:
-        spawn = False
+        pass
     ctx = mp.get_context("spawn" if spawn else "fork")
-    with ctx.Pool(workers) as pool:
+    pool = ctx.Pool(workers)
+    try:
         if log is None:
-            return pool.map(fn, jobs, chunksize=1)
-        import time as _time
-        out, t_last = [None] * len(jobs), _time.time()
-        for done, (i, r) in enumerate(pool.imap_unordered(_indexed, [(fn, i, j) for i, j in enumerate(jobs)]), 1):
-            out[i] = r
-            if _time.time() - t_last > 20 or done == len(jobs):
-                log(f"  {done}/{len(jobs)} jobs done")
-                t_last = _time.time()
+            out = pool.map(fn, jobs, chunksize=1)
+        else:
+            import time as _time
+            out, t_last = [None] * len(jobs), _time.time()
+            for done, (i, r) in enumerate(pool.imap_unordered(_indexed, [(fn, i, j) for i, j in enumerate(jobs)]),
+                                          1):
+                out[i] = r
+                if _time.time() - t_last > 20 or done == len(jobs):
+                    log(f"  {done}/{len(jobs)} jobs done")
+                    t_last = _time.time()
+        pool.close()
+        pool.join()
         return out
+    except BaseException:
+        pool.terminate()
+        pool.join()
+        raise
 
 
 def _indexed(args):
@@ -317,29 +330,64 @@ def _pack_from_arrays(meta, prefix, z):
     return pack
 
 
-def save_pack(path, pack):
+def matrix_fingerprint(kind, mats):
+    """Identity of the sparse operand(s) a pack was built for: the operation,
+    each matrix's shape and nnz, and a SHA-1 over each matrix's CSR indptr
+    and indices (int64).  Values are not hashed: a pack carries value ids, and
+    plans refresh values through them."""
+    import hashlib
+    lst = mats if isinstance(mats, (list, tuple)) else [mats]
+    h = hashlib.sha1()
+    shapes, nnzs = [], []
+    for A in lst:
+        A = _csr_of(A)
+        shapes.append([int(A.shape[0]), int(A.shape[1])])
+        nnzs.append(int(A.nnz))
+        h.update(np.ascontiguousarray(A.indptr, dtype=np.int64).tobytes())
+        h.update(np.ascontiguousarray(A.indices, dtype=np.int64).tobytes())
+    return {"kind": kind, "count": len(lst), "shapes": shapes, "nnz": nnzs, "sha1": h.hexdigest()}
+
+
+class StalePackError(ValueError):
+    """A saved pack was built for a different matrix than the one given."""
+
+
+def save_pack(path, pack, fingerprint=None):
     """Write an SpMM or SDDMM pack (the dicts of spmm_pack / sddmm_pack /
     the blocked variants), or a list of them (a batched plan's packs), to an
     .npz file: array fields as arrays, scalar fields and group counts as a
-    JSON header."""
+    JSON header, with the matrix fingerprint (matrix_fingerprint) if given."""
     import json
     arrays = {}
     if isinstance(pack, (list, tuple)):
         meta = {"list": [_pack_to_arrays(p, f"p{i}/", arrays) for i, p in enumerate(pack)]}
     else:
         meta = _pack_to_arrays(pack, "", arrays)
+    if fingerprint is not None:
+        meta["fingerprint"] = fingerprint
     arrays["__meta__"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez(path, **arrays)
 
 
-def load_pack(path):
-    """Read a pack (or list of packs) written by save_pack."""
+def load_pack(path, expect=None):
+    """Read a pack (or list of packs) written by save_pack.  expect: the
+    fingerprint of the matrix the caller will run the pack on; a file saved
+    with another fingerprint, or with none, raises StalePackError."""
     import json
     with np.load(path, allow_pickle=False) as z:
         meta = json.loads(bytes(z["__meta__"]).decode())
+        if expect is not None and meta.get("fingerprint") != expect:
+            got = meta.get("fingerprint")
+            raise StalePackError(f"{path}: pack built for {_fp_str(got)}, not for {_fp_str(expect)}")
         if "list" in meta:
             return [_pack_from_arrays(m, f"p{i}/", z) for i, m in enumerate(meta["list"])]
         return _pack_from_arrays(meta, "", z)
+
+
+def _fp_str(fp):
+    if not fp:
+        return "an unrecorded matrix"
+    return f"{fp['kind']} x{fp['count']} shapes {fp['shapes'][:2]} nnz {fp['nnz'][:2]} sha1 {fp['sha1'][:12]}"
 
 
 # ---------------------------------------------------------------------------
@@ -376,37 +424,109 @@ def _pack_any(kind, mats, opts, width, workers, log=None):
     return sddmm_pack_blocked(mats, opts, width, workers=workers, log=log)
 
 
+def _as_list(mats):
+    return list(mats) if isinstance(mats, (list, tuple)) else [mats]
+
+
+def _offsets(mats, axis):
+    return np.concatenate([[0], np.cumsum([_csr_of(A).shape[axis] for A in _as_list(mats)])]).astype(np.int64)
+
+
+def sample_rows(mats, rows):
+    """Rows `rows` (sorted global row ids) of the block-diagonal composition of
+    mats (one matrix or a list), as a float64 CSR over the global columns."""
+    lst = [_csr_of(A) for A in _as_list(mats)]
+    roff, coff = _offsets(lst, 0), _offsets(lst, 1)
+    parts = []
+    mi = np.searchsorted(roff, rows, side="right") - 1
+    for i in np.unique(mi):
+        local = np.asarray(rows)[mi == i] - roff[i]
+        sub = lst[i][local].tocsr()
+        parts.append(scipy.sparse.csr_matrix((sub.data.astype(np.float64), sub.indices.astype(np.int64) + coff[i],
+                                              sub.indptr), shape=(len(local), int(coff[-1]))))
+    return scipy.sparse.vstack(parts, format="csr")
+
+
+def spmm_sample_error(mats, B, C, samples=256, seed=0):
+    """Norm-wise relative error max|C - A.B| / max|A.B| over `samples` random
+    rows of C = A.B (A: mats, B and C device tensors), the products in
+    float64 on the host (the reference checks its kernel the same way before
+    timing it, my_fuse_formats.py:4194-4217)."""
+    import torch
+    M = int(C.shape[0])
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.choice(M, min(M, samples), replace=False))
+    S = sample_rows(mats, rows)
+    cols = np.unique(S.indices)
+    if len(cols) == 0:
+        got = C[torch.from_numpy(rows).to(C.device)].double().cpu().numpy()
+        return float(np.max(np.abs(got))) if got.size else 0.0
+    Bs = B[torch.from_numpy(cols).to(B.device)].double().cpu().numpy()
+    Sc = scipy.sparse.csr_matrix((S.data, np.searchsorted(cols, S.indices), S.indptr), shape=(len(rows), len(cols)))
+    ref = np.asarray(Sc @ Bs)
+    got = C[torch.from_numpy(rows).to(C.device)].double().cpu().numpy()
+    scale = np.max(np.abs(ref))
+    diff = float(np.max(np.abs(got - ref)))
+    return diff / scale if scale > 0 else diff
+
+
+def sddmm_sample_error(mats, X, Y, out, samples=4096, seed=0):
+    """Norm-wise relative error of the SDDMM output on `samples` random
+    nonzeros of the block-diagonal pattern, dot products in float64 on the
+    host (the reference's check against dgl, my_fuse_formats.py:4331-4354)."""
+    import torch
+    lst = [_csr_of(A) for A in _as_list(mats)]
+    roff, coff = _offsets(lst, 0), _offsets(lst, 1)
+    zoff = np.concatenate([[0], np.cumsum([A.nnz for A in lst])]).astype(np.int64)
+    nnz = int(zoff[-1])
+    if nnz == 0:
+        return 0.0
+    rng = np.random.default_rng(seed)
+    p = np.sort(rng.choice(nnz, min(nnz, samples), replace=False))
+    mi = np.searchsorted(zoff, p, side="right") - 1
+    grow, gcol = np.empty(len(p), np.int64), np.empty(len(p), np.int64)
+    for i in np.unique(mi):
+        sel = mi == i
+        q = p[sel] - zoff[i]
+        grow[sel] = np.searchsorted(lst[i].indptr, q, side="right") - 1 + roff[i]
+        gcol[sel] = lst[i].indices[q] + coff[i]
+    dev = X.device
+    xr = X[torch.from_numpy(grow).to(dev)].double().cpu().numpy()
+    yc = Y[torch.from_numpy(gcol).to(dev)].double().cpu().numpy()
+    ref = np.einsum("ij,ij->i", xr, yc)
+    got = out[torch.from_numpy(p).to(dev)].double().cpu().numpy()
+    scale = np.max(np.abs(ref))
+    diff = float(np.max(np.abs(got - ref)))
+    return diff / scale if scale > 0 else diff
+
+
+# the reference's accuracy bar for a candidate (assert_allclose rtol = atol =
+# 1e-2, my_fuse_formats.py:4204-4208, 4350-4354)
+TUNE_TOL = 1e-2
+
+
 def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps=20, workers=None, log=None,
               cache=None):
     """Build the plan of `mats` (one matrix, or a list for a batched plan)
-    under each candidate search setting and return (fastest plan, its pack,
+    under each candidate search setting, check each plan's output against a
+    float64 product on a sample (relative error <= TUNE_TOL, else the
+    candidate is dropped), time the rest and return (fastest plan, its pack,
     report).  kind: 'spmm' (width = N) or 'sddmm' (width = K_feat, FP32
     blocks).  report: {"choice": name, "ms": {name: ms per run},
-    "search_s": {name: seconds}}.
-    cache: an .npz path; if it exists its pack is used as is (no search, no
-    timing: report {"choice": "cached"}), else the chosen pack is saved there."""
+    "search_s": {name: seconds}, "err": {name: sampled relative error}}.
+    cache: an .npz path; if it exists and was saved for this very pattern
+    (matrix_fingerprint) its pack is used without search or timing (report
+    {"choice": "cached"}); a cache saved for another pattern raises
+    StalePackError.  Otherwise the chosen pack is saved there."""
     import time as _time
     import torch
     from . import _native
     if kind == "sddmm":
         blk_dtype = "float32"
-    if cache is not None and os.path.exists(cache):
-        pack = load_pack(cache)
-        plan = _native.Plan.spmm(pack, blk_dtype) if kind == "spmm" else _native.Plan.sddmm(pack, "float32")
-        return plan, pack, {"choice": "cached", "cache": str(cache)}
-    cands = candidates or tuning_candidates(blk_dtype, kind)
-    packs, search_s = {}, {}
-    for name, opts in cands.items():          # every search before the GPU is used here
-        t0 = _time.perf_counter()
-        packs[name] = _pack_any(kind, mats, opts, width, workers, log)
-        search_s[name] = round(_time.perf_counter() - t0, 3)
-        if log:
-            log(f"tune {kind}: searched {name} in {search_s[name]} s")
-    if isinstance(mats, (list, tuple)):
-        M = sum(A.shape[0] for A in mats)
-        K = sum(A.shape[1] for A in mats)
-    else:
-        M, K = mats.shape
+    fp = matrix_fingerprint(kind, mats)
+    make = (lambda pk: _native.Plan.spmm(pk, blk_dtype)) if kind == "spmm" else \
+        (lambda pk: _native.Plan.sddmm(pk, "float32"))
+    M, K = int(_offsets(mats, 0)[-1]), int(_offsets(mats, 1)[-1])
     dev = torch.device("cuda")
     g = torch.Generator(device=dev)
     g.manual_seed(0)
@@ -416,9 +536,42 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
     else:
         X = torch.randn(M, width, device=dev, generator=g)
         Y = torch.randn(K, width, device=dev, generator=g)
-    best, best_ms, times = None, float("inf"), {}
+
+    def check(plan, pack):
+        if kind == "spmm":
+            plan.run_spmm(Bm, out)
+            return spmm_sample_error(mats, Bm, out)
+        o = torch.empty(pack_nnz(pack), device=dev)
+        plan.run_sddmm(X, Y, o)
+        return sddmm_sample_error(mats, X, Y, o)
+
+    if cache is not None and os.path.exists(cache):
+        pack = load_pack(cache, expect=fp)
+        plan = make(pack)
+        err = check(plan, pack)
+        if not err <= TUNE_TOL:
+            plan.close()
+            raise StalePackError(f"{cache}: plan error {err:.3g} against float64 exceeds {TUNE_TOL}")
+        return plan, pack, {"choice": "cached", "cache": str(cache), "err": err}
+    cands = candidates or tuning_candidates(blk_dtype, kind)
+    packs, search_s = {}, {}
+    for name, opts in cands.items():          # every search before the GPU is used here
+        t0 = _time.perf_counter()
+        packs[name] = _pack_any(kind, mats, opts, width, workers, log)
+        search_s[name] = round(_time.perf_counter() - t0, 3)
+        if log:
+            log(f"tune {kind}: searched {name} in {search_s[name]} s")
+    best, best_ms, times, errs, choice = None, float("inf"), {}, {}, None
     for name, pack in packs.items():
-        plan = _native.Plan.spmm(pack, blk_dtype) if kind == "spmm" else _native.Plan.sddmm(pack, "float32")
+        plan = make(pack)
+        err = check(plan, pack)
+        errs[name] = float(f"{err:.4g}")
+        if not err <= TUNE_TOL:
+            if log:
+                log(f"tune {kind} {name}: dropped, error {err:.3g} against float64")
+            times[name] = None
+            plan.close()
+            continue
         if kind == "spmm":
             run = lambda: plan.run_spmm(Bm, out)
         else:
@@ -436,17 +589,19 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
         ms = s.elapsed_time(e) / reps
         times[name] = round(ms, 5)
         if log:
-            log(f"tune {kind} {name}: {ms * 1e3:.2f} us (search {search_s[name]} s)")
+            log(f"tune {kind} {name}: {ms * 1e3:.2f} us (search {search_s[name]} s, error {err:.2e})")
         if ms < best_ms:
             if best is not None:
                 best.close()
             best, best_ms, choice = plan, ms, name
         else:
             plan.close()
+    if best is None:
+        raise RuntimeError(f"tune {kind}: no candidate plan passed the float64 check ({errs})")
     if cache is not None:
         os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
-        save_pack(cache, packs[choice])
-    return best, packs[choice], {"choice": choice, "ms": times, "search_s": search_s}
+        save_pack(cache, packs[choice], fingerprint=fp)
+    return best, packs[choice], {"choice": choice, "ms": times, "search_s": search_s, "err": errs}
 
 
 def pack_nnz(pack):

@@ -42,8 +42,9 @@ def _tune_attention(ones, opts, workers, log, cache_dir, device):
     whose plan runs the fused kernel fastest on random Q, K, V."""
     import time as _time
     cache = None if cache_dir is None else os.path.join(cache_dir, "attention_fused.npz")
+    fp = api.matrix_fingerprint("attention", ones)
     if cache is not None and os.path.exists(cache):
-        return _native.Plan.attention(api.load_pack(cache)), {"choice": "cached", "cache": cache}
+        return _native.Plan.attention(api.load_pack(cache, expect=fp)), {"choice": "cached", "cache": cache}
     cands = api.tuning_candidates("bfloat16")
     packs, search_s = {}, {}
     for name, o in cands.items():
@@ -80,8 +81,22 @@ def _tune_attention(ones, opts, workers, log, cache_dir, device):
             plan.close()
     if cache is not None:
         os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
-        api.save_pack(cache, packs[choice])
+        api.save_pack(cache, packs[choice], fingerprint=fp)
     return best, {"choice": choice, "ms": times, "search_s": search_s}
+
+
+def _cached_packs(kind, mats, opts, workers, cache_dir, name):
+    """build_packs, read from / written to cache_dir/name when cache_dir is
+    given (with the masks' fingerprint), so a cached run starts no search."""
+    path = None if cache_dir is None else os.path.join(cache_dir, name)
+    fp = api.matrix_fingerprint(kind, mats)
+    if path is not None and os.path.exists(path):
+        return api.load_pack(path, expect=fp)
+    packs = build_packs(kind, mats, opts, workers)
+    if path is not None:
+        os.makedirs(cache_dir, exist_ok=True)
+        api.save_pack(path, packs, fingerprint=fp)
+    return packs
 
 
 class BatchedSparseAttention(object):
@@ -108,7 +123,8 @@ class BatchedSparseAttention(object):
             if tune:
                 self.att_plan, self.tuning = _tune_attention(ones, opts, workers, log, cache_dir, device)
             else:
-                packs = build_packs("spmm", self.masks, api.options_for(blk_dtype, opts), workers)
+                packs = _cached_packs("spmm", self.masks, api.options_for(blk_dtype, opts), workers, cache_dir,
+                                      "attention_fused_default.npz")
                 self.att_plan = _native.Plan.attention(packs)
         elif tune:
             # one search setting for all instances, chosen by measured latency
@@ -121,8 +137,10 @@ class BatchedSparseAttention(object):
                                                      cache=cp("sddmm.npz"))
             self.tuning = {"spmm": rep, "sddmm": srep}
         else:
-            spmm_packs = build_packs("spmm", self.masks, api.options_for(blk_dtype, opts), workers)
-            sddmm_packs = build_packs("sddmm", self.masks, api.options_for("float32", opts), workers)
+            spmm_packs = _cached_packs("spmm", self.masks, api.options_for(blk_dtype, opts), workers, cache_dir,
+                                       "unfused_spmm.npz")
+            sddmm_packs = _cached_packs("sddmm", self.masks, api.options_for("float32", opts), workers, cache_dir,
+                                        "unfused_sddmm.npz")
             self.spmm_plan = _native.Plan.spmm(spmm_packs, blk_dtype)
             self.sddmm_plan = _native.Plan.sddmm(sddmm_packs, "float32")
         big = scipy.sparse.block_diag(self.masks, format="csr")

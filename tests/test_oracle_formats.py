@@ -175,3 +175,88 @@ def test_compute_tile_known_answers():
     assert len(ell.idx_values_list[0][0]) == 6 and ell.position_space[0].shape[0] == 8
     te = gf.ComputeTile(ell, ((None, 8), (None, 32), (1, 4)), (0, 0, 0), {})
     assert (te.tile_i_rng, te.tile_k_rng, te.nnz, te.j_num, te.nnz_uncovered) == ((0, 7), (0, 3), 16, 32, 512)
+
+
+# --- tile-shape candidates (gen_formats_v2.py:1188-1281, 1926-2068) ----------
+
+def _shape(I, K):
+    return ((None, I), (None, 32), (1, K))
+
+
+def test_ell_tile_shapes_known_answer():
+    # mbs = 32: I*K = 256 with K = 1..32, I*32 a multiple of 128 and >= 256;
+    # product order is I ascending (itertools.product over I, then K)
+    assert of.ell_tb_tile_shapes(1000, 32) == [_shape(8, 32), _shape(16, 16), _shape(32, 8), _shape(64, 4),
+                                               _shape(128, 2), _shape(256, 1)]
+    # fewer than 256 split rows: I <= 2^floor(log2(100)) = 64
+    assert of.ell_tb_tile_shapes(100, 32) == [_shape(8, 32), _shape(16, 16), _shape(32, 8), _shape(64, 4)]
+    # the tiny matrix's ELL ops: 8 pieces (mbs 4) and 11 pieces (mbs 2)
+    assert of.ell_tb_tile_shapes(8, 4) == [_shape(8, 4)]
+    assert of.ell_tb_tile_shapes(11, 2) == [_shape(8, 2)]
+    # mbs 8: I*K = 64 with K = 1, 2, 4, 8; I = 64 / K <= 64 pieces
+    assert of.ell_tb_tile_shapes(64, 8) == [_shape(8, 8), _shape(16, 4), _shape(32, 2), _shape(64, 1)]
+    # 63 pieces: I <= 32, so K = 1 drops out
+    assert of.ell_tb_tile_shapes(63, 8) == [_shape(8, 8), _shape(16, 4), _shape(32, 2)]
+
+
+def test_tc_tile_sizes_known_answer():
+    spmm = of.tc_tile_sizes('spmm')
+    assert [ts[2][1] for ts in spmm] == [160, 144, 128, 112, 96, 80, 64, 48, 32, 16]
+    assert all(ts[0] == (None, 16) and ts[1] == (None, 32) for ts in spmm)
+    assert of.tc_tile_sizes('spmm', k_notsorted=True) == [((None, 16), (None, 32), (None, 16))]
+    assert of.tc_tile_sizes('sddmm') == [((None, 16), (None, 16), (None, 16)), ((None, 16), (None, 32), (None, 16)),
+                                         ((None, 16), (None, 64), (None, 16))]
+    assert of.sddmm_tc_tile_sizes(32) == [((None, 16), (None, 32), (None, 16))]
+    assert of.sddmm_tc_tile_sizes(512) == [((None, 16), (None, 64), (None, 16))]
+    assert of.sddmm_tc_tile_sizes(16) == [((None, 16), (None, 16), (None, 16))]
+
+
+def test_tile_params_known_answer():
+    assert of.tile_params('sparse_template_ell', _shape(8, 32), 32) == [
+        {'use_implicit_unroll': True, 'last_bucket': True}]
+    assert of.tile_params('sparse_template_ell', _shape(16, 16), 32) == [
+        {'use_implicit_unroll': True, 'last_bucket': False}]
+    assert of.tile_params('TensorCore_template', ((None, 16), (None, 32), (None, 64)), 32) == [
+        {'use_implicit_unroll': True, 'mma_shape_str': 'm16n16k16'}]
+    assert of.tile_params('1D_sddmm', (32,), 32) == [
+        {'tx': 8, 'ty': 4, 'vec_size': 4, 'group_size': 8, 'max_bucket_size': 32}]
+    assert of.tile_params('TC_sddmm', ((None, 16), (None, 32), (None, 16)), 32) == [
+        {'mma_shape_str': 'm16n16k16', 'warp_num': 1, 'vec_size': 4}]
+
+
+@pytest.mark.parametrize("name,A", _mats())
+@pytest.mark.parametrize("mbs", [2, 4, 8, 32])
+def test_product_tile_shapes_match_oracle(name, A, mbs):
+    op = gf.Op('spmm', [0, 1], [2], [A.shape[0], 4, A.shape[1]], None,
+               [A, gf.DenseTensor(np.zeros((A.shape[1], 4), np.float32))], None)
+    gf.set_config(gf.SearchConfig())
+    ops, _ = gf.gen_candidate_formats(op, mbs, ([1], [1], [1]), False)
+    ell, tc = ops[0], ops[-1]
+    shapes = gf.ell_tile_shapes(ell, mbs)
+    assert shapes == of.ell_tb_tile_shapes(len(ell.hyb_new_rows[0]), mbs)
+    for ts in shapes:
+        assert gf.get_params_list(ell, ts, mbs, is_tb_tile=True) == of.tile_params('sparse_template_ell', ts, mbs)
+    assert gf.get_tile_sizes_idx(tc, mbs, is_tb_tile=True) == of.tc_tile_sizes('spmm')
+    for ts in of.tc_tile_sizes('spmm'):
+        assert gf.get_params_list(tc, ts, mbs, is_tb_tile=True) == of.tile_params('TensorCore_template', ts, mbs)
+
+
+@pytest.mark.parametrize("kf", [16, 32, 64, 128])
+def test_product_sddmm_tile_shapes_match_oracle(kf):
+    A = tiny()
+    op = gf.Op('sddmm', [0, 1], [2], [A.shape[0], kf, A.shape[1]], None,
+               [A, gf.DenseTensor(np.zeros((A.shape[0], kf), np.float32)),
+                gf.DenseTensor(np.zeros((A.shape[1], kf), np.float32))], None)
+    cfg = gf.SearchConfig()
+    cfg.op_type = 'sddmm'
+    gf.set_config(cfg)
+    try:
+        ops, _ = gf.gen_candidate_formats(op, 32, ([1], [1], [1]), False)
+        tc = [o for o in ops if gf.get_template_str(o) == 'TC_sddmm'][0]
+        one_d = [o for o in ops if gf.get_template_str(o) == '1D_sddmm'][0]
+        assert gf.get_tile_sizes_idx(tc, 32) == of.sddmm_tc_tile_sizes(kf)
+        assert gf.get_params_list(one_d, (32,), 32) == of.tile_params('1D_sddmm', (32,), 32)
+        assert gf.get_params_list(tc, of.sddmm_tc_tile_sizes(kf)[0], 32) == of.tile_params(
+            'TC_sddmm', of.sddmm_tc_tile_sizes(kf)[0], 32)
+    finally:
+        gf.set_config(gf.SearchConfig())

@@ -233,3 +233,48 @@ def test_load_snap_graph_relabels_dedupes_and_pads(tmp_path):
     assert op2.idx_lens == [128, 8, 160]
     op3 = load_snap_graph('spmm', str(p), feat_size=8, pad=False)
     assert op3.idx_lens == [3, 8, 3]
+
+
+def test_pack_cache_refuses_another_pattern(tmp_path):
+    A = datasets.pruned_layer(seed=1, M=64, K=48, N=8)[0]
+    B = A.copy().tolil()
+    B[0, 0] = 0.0 if A[0, 0] != 0 else 1.0           # one nonzero moved in or out
+    B = scipy.sparse.csr_matrix(B)
+    B.eliminate_zeros()
+    pack = api.spmm_pack(A, api.SearchOptions())[0]
+    fa = api.matrix_fingerprint("spmm", A)
+    api.save_pack(tmp_path / "p.npz", pack, fingerprint=fa)
+    assert api.load_pack(tmp_path / "p.npz", expect=fa)["M"] == 64
+    # same shape, a different pattern: refused
+    with pytest.raises(api.StalePackError):
+        api.load_pack(tmp_path / "p.npz", expect=api.matrix_fingerprint("spmm", B))
+    # same pattern, other operation: refused
+    with pytest.raises(api.StalePackError):
+        api.load_pack(tmp_path / "p.npz", expect=api.matrix_fingerprint("sddmm", A))
+    # a file saved without a fingerprint is refused when one is expected
+    api.save_pack(tmp_path / "q.npz", pack)
+    with pytest.raises(api.StalePackError):
+        api.load_pack(tmp_path / "q.npz", expect=fa)
+    # values do not enter the fingerprint (plans refresh values by id)
+    A2 = A.copy()
+    A2.data = A2.data * 3
+    assert api.matrix_fingerprint("spmm", A2) == fa
+
+
+def test_sampled_error_checks_detect_a_wrong_row():
+    import torch
+    mats = [datasets.pruned_layer(seed=s, M=48, K=32, N=8)[0] for s in (2, 3)]
+    big = scipy.sparse.block_diag(mats, format="csr")
+    rng = np.random.default_rng(0)
+    B = torch.from_numpy(rng.standard_normal((64, 5)).astype(np.float32))
+    C = torch.from_numpy(np.asarray(big @ B.numpy().astype(np.float64)).astype(np.float32))
+    assert api.spmm_sample_error(mats, B, C, samples=96) < 1e-6
+    C[70, 3] += 1.0
+    assert api.spmm_sample_error(mats, B, C, samples=96) > 1e-2
+    X = torch.from_numpy(rng.standard_normal((96, 4)).astype(np.float32))
+    Y = torch.from_numpy(rng.standard_normal((64, 4)).astype(np.float32))
+    rows = np.repeat(np.arange(96), np.diff(big.indptr))
+    out = torch.from_numpy((X.numpy()[rows] * Y.numpy()[big.indices]).sum(1))
+    assert api.sddmm_sample_error(mats, X, Y, out, samples=big.nnz) < 1e-6
+    out[big.nnz - 1] += 1.0
+    assert api.sddmm_sample_error(mats, X, Y, out, samples=big.nnz) > 1e-2
