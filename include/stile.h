@@ -216,9 +216,35 @@ int stile_plan_create_sddmm(const StileSddmmPack* pack, int32_t blk_dtype,
                             StilePlan** out);
 
 /* C[M][N] = A * B with the plan's tile set.  B: K x N FP32 (row stride ldb),
- * C: M x N FP32 (row stride ldc).  Every row of C is written. */
+ * C: M x N FP32 (row stride ldc).  Every row of C is written.  Same as
+ * stile_spmm_run_ex(plan, STILE_BLK_F32, B, ldb, N, C, ldc, 0, NULL, 0,
+ * stream). */
 int stile_spmm_run(const StilePlan* plan, const float* B, int64_t ldb,
                    int32_t N, float* C, int64_t ldc, void* stream);
+
+/* Run flag: sum the pieces of rows with more than 256 sparse nonzeros, and
+ * the parts of windows with more than 256 dense chunks, in a fixed order
+ * through the caller's workspace instead of with atomicAdd in completion
+ * order: results are then bit-identical run to run.  (Without it only those
+ * rows can differ in their last bits between runs.) */
+#define STILE_RUN_DETERMINISTIC 1
+
+/* Bytes of workspace stile_spmm_run_ex needs for width N and these flags
+ * (0 without STILE_RUN_DETERMINISTIC, or when the plan has no split rows or
+ * huge windows). */
+int stile_spmm_workspace_bytes(const StilePlan* plan, int32_t N, int32_t flags, int64_t* bytes);
+
+/* C[M][N] = A * B with B's elements of type b_dtype (STILE_BLK_F32, _BF16 or
+ * _F16: the reference keeps B in the run dtype, FP16 by default,
+ * MY_sparse/my_fuse_formats.py:3529).  B: K x N with row stride ldb
+ * (elements); C: M x N FP32.  The sparse (1D/ELL) products are FP32 on B's
+ * exact values; the dense blocks run on the plan's MFMA type (a 16-bit B
+ * staged for BF16 blocks is rounded to BF16 only if it is FP16 or FP32).
+ * flags: 0 or STILE_RUN_DETERMINISTIC; workspace: a device buffer of at
+ * least stile_spmm_workspace_bytes bytes, 16-byte aligned (may be NULL when
+ * that is 0).  Launches kernels only (no allocation, no synchronisation). */
+int stile_spmm_run_ex(const StilePlan* plan, int32_t b_dtype, const void* B, int64_t ldb, int32_t N, float* C,
+                      int64_t ldc, int32_t flags, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* out[p] = <X[row(p)], Y[col(p)]> for every nonzero p of the plan's pattern;
  * X: M x Kf, Y: N x Kf FP32 with row strides ldx, ldy; out has pack->nnz

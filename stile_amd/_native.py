@@ -12,6 +12,7 @@ import numpy as np
 from . import build as _build
 
 STILE_BLK_F32, STILE_BLK_BF16, STILE_BLK_F16 = 0, 1, 2
+STILE_RUN_DETERMINISTIC = 1
 _BLK_NAMES = {"float32": STILE_BLK_F32, "fp32": STILE_BLK_F32, "f32": STILE_BLK_F32,
               "bfloat16": STILE_BLK_BF16, "bf16": STILE_BLK_BF16,
               "float16": STILE_BLK_F16, "fp16": STILE_BLK_F16, "f16": STILE_BLK_F16}
@@ -92,6 +93,11 @@ _SIGNATURES = {
                                                        ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     "stile_spmm_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                       ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+    "stile_spmm_run_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+    "stile_spmm_workspace_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.POINTER(ctypes.c_int64)]),
     "stile_sddmm_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                        ctypes.c_void_p, ctypes.c_void_p]),
@@ -440,21 +446,44 @@ class Plan:
         _check(load().stile_plan_info(self._h, ctypes.byref(inf)))
         return {f: getattr(inf, f) for f, _ in StilePlanInfo._fields_}
 
-    def run_spmm(self, B, out=None):
+    def workspace_bytes(self, N, deterministic=False):
+        """Device workspace (bytes) run_spmm needs at width N."""
+        n = ctypes.c_int64()
+        _check(load().stile_spmm_workspace_bytes(self._h, int(N), STILE_RUN_DETERMINISTIC if deterministic else 0,
+                                                 ctypes.byref(n)))
+        return n.value
+
+    def run_spmm(self, B, out=None, deterministic=False, workspace=None):
+        """C = A.B.  B: float32, bfloat16 or float16 (K x N, rows contiguous);
+        C float32.  deterministic: split rows and huge windows are summed in a
+        fixed order through `workspace` (a device uint8/any tensor of at least
+        workspace_bytes(N, True) bytes; allocated here when not given)."""
         import torch
         if self.kind != "spmm":
             raise StileError("not an SpMM plan")
         _dev_check(B)
-        if B.dtype != torch.float32:
-            raise StileError("B must be float32")
+        codes = {torch.float32: STILE_BLK_F32, torch.bfloat16: STILE_BLK_BF16, torch.float16: STILE_BLK_F16}
+        if B.dtype not in codes:
+            raise StileError("B must be float32, bfloat16 or float16")
         if B.shape[0] != self.K:
             raise StileError(f"B has {B.shape[0]} rows, the plan's A has {self.K} columns")
         if B.stride(1) != 1:
             B = B.contiguous()
         N = B.shape[1]
         C = _out(out, (self.M, N), B.device)
-        _check(load().stile_spmm_run(self._h, B.data_ptr(), B.stride(0), N, C.data_ptr(), C.stride(0),
-                                     _stream(B)))
+        flags, ws_ptr, ws_bytes = 0, None, 0
+        if deterministic:
+            flags = STILE_RUN_DETERMINISTIC
+            need = self.workspace_bytes(N, True)
+            if need > 0:
+                if workspace is None:
+                    workspace = torch.empty(need, dtype=torch.uint8, device=B.device)
+                _dev_check(workspace)
+                if workspace.device != B.device or not workspace.is_contiguous():
+                    raise StileError("workspace must be a contiguous tensor on B's device")
+                ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+        _check(load().stile_spmm_run_ex(self._h, codes[B.dtype], B.data_ptr(), B.stride(0), N, C.data_ptr(),
+                                        C.stride(0), flags, ws_ptr, ws_bytes, _stream(B)))
         return C
 
     def run_sddmm(self, X, Y, out=None):

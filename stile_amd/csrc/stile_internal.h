@@ -28,8 +28,11 @@ constexpr int kBigWaves = 8;
 // rest of the grid.  Windows up to kHugeChunks keep the fixed-order sums.
 constexpr int kHugeChunks = 256;
 // BF16 plans whose matrices all have K <= kLdsMaxK run their dense windows on
-// the B-stationary kernel (B rows of one matrix staged in LDS, K * 128 bytes).
+// the B-stationary kernel (B rows of one matrix staged in LDS, K rows of
+// kLdsRowBytes: 64 BF16 columns and 32 bytes of padding, so K <= 1024 fits
+// the 160 KiB of LDS of a gfx950 CU).
 constexpr int kLdsMaxK = 1024;
+constexpr int kLdsRowBytes = 160;
 // Work units of the B-stationary kernel: kLdsLists lists of about
 // kLdsTargets[t] units each; the plan picks one per column-chunk count
 // nC <= kLdsPickMax by a makespan estimate (see build_lds_units).
@@ -74,10 +77,19 @@ struct SpmmPlanDev {
   const int32_t* part_c1;
   int32_t nchunks;
   int32_t blk_dtype;
-  int32_t npieces;           // split pieces of long sparse rows
+  int32_t npieces;           // split pieces of long sparse rows (in slot order)
+  int32_t npieces_dense;     // pieces [0, npieces_dense) belong to slots of dense windows
   const int32_t* piece_row;  // [npieces] output row
   const int32_t* piece_beg;  // [npieces] range in res_cols/res_vals
   const int32_t* piece_end;
+  // split rows (deterministic mode): row r's pieces are [srow_p0[r], srow_p1[r])
+  int32_t nsrows;
+  int32_t nsrows_dense;      // split rows [0, nsrows_dense) are slots of dense windows
+  const int32_t* srow_row;
+  const int32_t* srow_p0;
+  const int32_t* srow_p1;
+  // parts of huge window w (deterministic mode): [hw_p0[w], hw_p0[w + 1])
+  const int32_t* hw_p0;      // [W_huge + 1]
   const int32_t* win_rows;   // [W*16]
   const int32_t* tc_ptr;     // [W+1]
   const int32_t* tc_cols;    // [nchunks*32]
@@ -139,8 +151,13 @@ hipError_t launch_csr_sddmm_f32(int M, int nnz, int Kf, const int32_t* indptr,
                                 float* out, hipStream_t s);
 // One-time kernel attributes (dynamic LDS above 64 KiB) for a plan.
 void prepare_spmm_kernels(const SpmmPlanDev& p);
-hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb,
-                           int N, float* C, int64_t ldc, hipStream_t s);
+// b_dtype: STILE_BLK_F32 / BF16 / F16 (the type of B's elements).
+// deterministic: split rows and huge windows sum their pieces in a fixed
+// order through ws (workspace_floats(p, N) floats) instead of atomically.
+hipError_t launch_hyb_spmm(const SpmmPlanDev& p, int b_dtype, const void* B, int64_t ldb,
+                           int N, float* C, int64_t ldc, bool deterministic, float* ws, hipStream_t s);
+// Floats of workspace the deterministic mode needs at width N.
+int64_t spmm_workspace_floats(const SpmmPlanDev& p, int N);
 hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,
                             int64_t ldx, const float* Y, int64_t ldy, int Kf,
                             float* out, hipStream_t s);

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 #include <cstdlib>
 #include <cstring>
 
@@ -51,21 +52,44 @@ __device__ __forceinline__ void neu_add(float& s, float& c, float x) {
   s = t;
 }
 
-// Load 4 consecutive floats of row k of B starting at column col; columns at
-// or beyond N read as 0.  vec: B rows are 16-byte aligned and N % 4 == 0.
-__device__ __forceinline__ float4 load_b4(const float* __restrict__ B,
-                                          int64_t ldb, int k, int col, int N,
-                                          bool vec) {
-  const float* p = B + (int64_t)k * ldb + col;
-  if (vec) {
-    if (col < N) return ld4(p);
-    return make_float4(0.f, 0.f, 0.f, 0.f);
+// Elements of B: FP32, BF16 or FP16.  load_b4 returns four consecutive
+// columns of row k starting at column col as FP32 (exact for the 16-bit
+// types); columns at or beyond N read as 0.  vec: B rows are aligned for one
+// vector load of four elements (16 bytes FP32, 8 bytes 16-bit) and N % 4 == 0.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
+
+template <typename TB>
+__device__ __forceinline__ float4 cvt4(uint2 w) {
+  if constexpr (std::is_same<TB, __bf16>::value) {
+    return make_float4(bf16_lo(w.x), bf16_hi(w.x), bf16_lo(w.y), bf16_hi(w.y));
+  } else {
+    const f16x2 a = __builtin_bit_cast(f16x2, w.x), b = __builtin_bit_cast(f16x2, w.y);
+    return make_float4((float)a[0], (float)a[1], (float)b[0], (float)b[1]);
+  }
+}
+
+template <typename TB>
+__device__ __forceinline__ float4 load_b4(const TB* __restrict__ B, int64_t ldb, int k, int col, int N, bool vec) {
+  const TB* p = B + (int64_t)k * ldb + col;
+  if constexpr (std::is_same<TB, float>::value) {
+    if (vec) {
+      if (col < N) return ld4(p);
+      return make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  } else {
+    if (vec) {
+      if (col < N) return cvt4<TB>(*reinterpret_cast<const uint2*>(p));
+      return make_float4(0.f, 0.f, 0.f, 0.f);
+    }
   }
   float4 r;
-  r.x = col < N ? p[0] : 0.f;
-  r.y = col + 1 < N ? p[1] : 0.f;
-  r.z = col + 2 < N ? p[2] : 0.f;
-  r.w = col + 3 < N ? p[3] : 0.f;
+  r.x = col < N ? (float)p[0] : 0.f;
+  r.y = col + 1 < N ? (float)p[1] : 0.f;
+  r.z = col + 2 < N ? (float)p[2] : 0.f;
+  r.w = col + 3 < N ? (float)p[3] : 0.f;
   return r;
 }
 
@@ -299,36 +323,42 @@ hipError_t launch_csr_sddmm_f32(int M, int nnz, int Kf, const int32_t* indptr,
 // One wave = (window w, 64 output columns n0..n0+63).  Lane l = 16g + j owns
 // output columns n0 + 4j .. n0 + 4j + 3 of rows 4g .. 4g+3 of the window: the
 // four 16x16 MFMA tiles t = 0..3 of the wave hold column n0 + 4j + t in MFMA
-// column j, so every B load and every C store is a float4 and MFMA output
-// element (row 4g+i, column j) of tile t is exactly the lane's own element.
+// column j, so every B load and every C store is four consecutive columns and
+// MFMA output element (row 4g+i, column j) of tile t is exactly the lane's own
+// element.
 //
 // Dense chunk (32 k-slots): k-slot 4e + g of lane l is column tc_cols[c][g][e]
-// of A; the lane loads B row tc_cols[c][g][e], columns n0+4j..+3, for e = 0..7.
-//   BF16: one v_mfma_f32_16x16x32_bf16 per tile (operand element e of lane l
-//         is k-slot 4e + g for both A and B; the pairing is what matters).
+// of A, A operand element e of lane l (the pairing of k-slots with operand
+// elements is the same for A and B, which is all the MFMA needs).
+//   BF16: one v_mfma_f32_16x16x32_bf16 per tile;
+//   FP16: one v_mfma_f32_16x16x32_f16 per tile, times the chunk's scale;
 //   FP32: eight v_mfma_f32_16x16x4_f32 per tile (MFMA e reduces k-slots
 //         4e + 0..3; lane group g supplies k-slot 4e + g).
-// Sparse part: the lane adds v * B[col][n0+4j..+3] for the first kSplitNnz
-// FP32 nonzeros of its rows (compensated when a row has more than kHubNnz);
-// the rest of a long row is added by hyb_spmm_fixup_kernel.
+// Sparse part: the lane adds v * B[col][n0+4j..+3] for the FP32 nonzeros of
+// its rows (compensated when a row has more than kHubNnz).
 //
-// Kernels of one run, in stream order: the dense windows [0, W_tc) store
-// their dense parts (LDS kernel, or the streaming kernel with one wave per
-// window and 64 columns, a workgroup for windows of more than kBigChunks
-// chunks); the sparse kernel then adds the sparse parts of those windows'
-// rows and writes the sparse-only windows (one row per lane group); the
-// fix-up kernel adds the pieces of long rows.
+// B is FP32, BF16 or FP16 (TB); every product and sum of the sparse part is
+// FP32 on B's exact values.
+//
+// Kernels of one run, in stream order.  Plans on the B-stationary kernel
+// (BF16 blocks, K <= kLdsMaxK): that kernel computes every dense window with
+// its sparse part, and as its tail (when they are few) the sparse-only rows;
+// otherwise the sparse kernel runs the sparse-only rows.  Other plans: the
+// streaming kernel stores the dense windows (a workgroup for windows of more
+// than kBigChunks chunks), the sparse kernel adds the first kSplitNnz sparse
+// nonzeros of those windows' rows and writes the sparse-only windows.  Last,
+// the pieces of rows longer than kSplitNnz: added with atomicAdd, or, in the
+// deterministic mode, summed into a workspace and added in piece order.
 // ---------------------------------------------------------------------------
 
-// Sum of v * B[col][n0+4j..+3] over nonzeros [r0, r1) of the sparse part.
 // Gathers in flight per lane group in the sparse and mixed slots.
 constexpr int kSparseDepth = 8;
 
-
+// Sum of v * B[col][n0+4j..+3] over nonzeros [r0, r1) of the sparse part.
 // DEPTH 8 keeps eight gathers in flight with the next ids prefetched; DEPTH 4
 // (four at a time) is for kernels whose register budget is tight.
-template <int DEPTH>
-__device__ __forceinline__ float4 sparse_row_sum(const SpmmPlanDev& p, const float* __restrict__ B,
+template <int DEPTH, typename TB>
+__device__ __forceinline__ float4 sparse_row_sum(const SpmmPlanDev& p, const TB* __restrict__ B,
                                                  int64_t ldb, int N, int col, bool vec, int r0, int r1,
                                                  bool compensated) {
   float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -422,8 +452,7 @@ __device__ __forceinline__ SlotRows load_rows(const SpmmPlanDev& p, int slot0) {
 }
 
 // Store the dense accumulators of the lane group's 4 rows (acc[t][i] is row
-// 4g+i, column col+t).  The sparse parts of these rows, if any, are added
-// afterwards by hyb_spmm_sparse_kernel.
+// 4g+i, column col+t).
 template <bool NT = false>
 __device__ __forceinline__ void store_window(float* __restrict__ C, int64_t ldc, int N, int col, bool vec,
                                              const SlotRows& sr, const f32x4 (&acc)[4]) {
@@ -434,8 +463,9 @@ __device__ __forceinline__ void store_window(float* __restrict__ C, int64_t ldc,
 }
 
 // Slot of a dense window with a sparse part: C[row] += first kSplitNnz
-// nonzeros of the part (the dense kernel stored C[row] before).
-__device__ __forceinline__ void add_slot(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb, int N,
+// nonzeros of the part (the streaming dense kernel stored C[row] before).
+template <typename TB>
+__device__ __forceinline__ void add_slot(const SpmmPlanDev& p, const TB* __restrict__ B, int64_t ldb, int N,
                                          float* __restrict__ C, int64_t ldc, int col, bool vec, int slot) {
   const int row = p.win_rows[slot];
   const int r0 = p.res_ptr[slot], rend = p.res_ptr[slot + 1];
@@ -446,21 +476,15 @@ __device__ __forceinline__ void add_slot(const SpmmPlanDev& p, const float* __re
 }
 
 // One slot (sparse-only windows, one row per lane group).
-__device__ __forceinline__ void finish_slot(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
+template <typename TB>
+__device__ __forceinline__ void finish_slot(const SpmmPlanDev& p, const TB* __restrict__ B, int64_t ldb,
                                             int N, float* __restrict__ C, int64_t ldc, int col, bool vec,
-                                            int slot, float4 dense) {
+                                            int slot) {
   const int row = p.win_rows[slot];
   const int r0 = p.res_ptr[slot], rend = p.res_ptr[slot + 1];
   const int r1 = min(rend, r0 + kSplitNnz);
-  float4 s = sparse_row_sum<kSparseDepth>(p, B, ldb, N, col, vec, r0, r1, rend - r0 > kHubNnz);
-  if (row >= 0) {
-    float4 o;
-    o.x = dense.x + s.x;
-    o.y = dense.y + s.y;
-    o.z = dense.z + s.z;
-    o.w = dense.w + s.w;
-    store_c4(C, ldc, row, col, N, vec, o);
-  }
+  const float4 s = sparse_row_sum<kSparseDepth>(p, B, ldb, N, col, vec, r0, r1, rend - r0 > kHubNnz);
+  if (row >= 0) store_c4(C, ldc, row, col, N, vec, s);
 }
 
 template <int MODE>
@@ -499,9 +523,10 @@ __device__ __forceinline__ void mfma_chunk(const SpmmPlanDev& p, int64_t c, int 
   }
 }
 
-// B rows of one chunk for lane group g: 8 float4 loads (addresses from the
-// chunk's column ids, already in registers).
-__device__ __forceinline__ void load_chunk_b(const float* __restrict__ B, int64_t ldb, int col, int N, bool vec,
+// B rows of one chunk for lane group g: 8 loads of four columns (addresses
+// from the chunk's column ids, already in registers).
+template <typename TB>
+__device__ __forceinline__ void load_chunk_b(const TB* __restrict__ B, int64_t ldb, int col, int N, bool vec,
                                              const int4& ka, const int4& kb, float4 (&b)[8]) {
   b[0] = load_b4(B, ldb, ka.x, col, N, vec);
   b[1] = load_b4(B, ldb, ka.y, col, N, vec);
@@ -517,8 +542,8 @@ __device__ __forceinline__ void load_chunk_b(const float* __restrict__ B, int64_
 // independent B loads in flight) and the column ids of the next two chunks
 // fetched before the current step's B loads, so a step waits for one memory
 // latency instead of two.
-template <int MODE>
-__device__ __forceinline__ void window_dense(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
+template <int MODE, typename TB>
+__device__ __forceinline__ void window_dense(const SpmmPlanDev& p, const TB* __restrict__ B, int64_t ldb,
                                              int N, int col, bool vec, int c0, int c1, int lane,
                                              f32x4 (&acc)[4]) {
   const int g = lane >> 4;
@@ -550,8 +575,8 @@ __device__ __forceinline__ void window_dense(const SpmmPlanDev& p, const float* 
   }
 }
 
-template <int MODE>
-__device__ __forceinline__ void hyb_spmm_window(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
+template <int MODE, typename TB>
+__device__ __forceinline__ void hyb_spmm_window(const SpmmPlanDev& p, const TB* __restrict__ B, int64_t ldb,
                                                 int N, float* __restrict__ C, int64_t ldc, int w, int n0,
                                                 bool vec) {
   const int lane = threadIdx.x & 63;
@@ -567,30 +592,39 @@ __device__ __forceinline__ void hyb_spmm_window(const SpmmPlanDev& p, const floa
   store_window(C, ldc, N, col, vec, sr, acc);
 }
 
-// Blocks [0, n_parts * nC) each run one (part of a huge window, 64 columns)
-// and add into C with atomicAdd; the next big_blocks each run one (big
-// window, 64 columns): wave v takes
-// the v-th contiguous share of the window's chunks, waves 1.. leave their
-// partial sums in LDS and wave 0 adds them in wave order, then adds the
-// sparse parts and stores the 16 rows.
-// The other blocks run kBigWaves waves each over the remaining dense windows
-// [W_big, W_tc), one wave per window and 64 columns.
-template <int MODE>
-__global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p, const float* __restrict__ B,
+// Blocks [0, n_parts * nC) each run one (part of a huge window, 64 columns):
+// they add their partial rows into C with atomicAdd (C rows zeroed before),
+// or, with ws (deterministic mode), store them to ws[part][16][N] for
+// hyb_spmm_part_reduce_kernel.  The next big_blocks each run one (big window,
+// 64 columns): wave v takes the v-th contiguous share of the window's chunks,
+// waves 1.. leave their partial sums in LDS and wave 0 adds them in wave
+// order and stores the 16 rows.  The other blocks run kBigWaves waves each
+// over the remaining dense windows [W_big, W_tc), one wave per window and 64
+// columns.
+template <int MODE, typename TB>
+__global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p, const TB* __restrict__ B,
                                                                   int64_t ldb, int N, float* __restrict__ C,
                                                                   int64_t ldc, int nC, int big_blocks,
-                                                                  bool vec) {
+                                                                  bool vec, float* __restrict__ ws) {
   __shared__ f32x4 part[kBigWaves - 1][4][64];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int huge_blocks = p.n_parts * nC;
-  if ((int)blockIdx.x < huge_blocks) {
-    // one part of a huge window: the partial rows are added into C (zeroed
-    // before this kernel) with atomicAdd
-    const int pt = blockIdx.x / nC;
-    const int w = p.part_win[pt];
-    const int col = (blockIdx.x - pt * nC) * 64 + 4 * (lane & 15);
-    const int c0 = p.part_c0[pt], c1 = p.part_c1[pt];
+  const bool in_huge = (int)blockIdx.x < huge_blocks;
+  if (in_huge || (int)blockIdx.x < huge_blocks + big_blocks) {
+    int w, col, c0, c1, pt = 0;
+    if (in_huge) {
+      pt = blockIdx.x / nC;
+      w = p.part_win[pt];
+      col = (blockIdx.x - pt * nC) * 64 + 4 * (lane & 15);
+      c0 = p.part_c0[pt];
+      c1 = p.part_c1[pt];
+    } else {
+      w = p.W_huge + (int)(blockIdx.x - huge_blocks) / nC;
+      col = (int)(blockIdx.x - huge_blocks - (w - p.W_huge) * nC) * 64 + 4 * (lane & 15);
+      c0 = p.tc_ptr[w];
+      c1 = p.tc_ptr[w + 1];
+    }
     const int share = (c1 - c0 + kBigWaves - 1) / kBigWaves;
     const int a = min(c0 + wv * share, c1), b = min(a + share, c1);
     SlotRows sr;
@@ -604,45 +638,31 @@ __global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p,
       for (int t = 0; t < 4; ++t) part[wv - 1][t][lane] = acc[t];
     }
     __syncthreads();
-    if (wv == 0) {
-      for (int v = 0; v < kBigWaves - 1; ++v) {
+    if (wv != 0) return;
+    for (int v = 0; v < kBigWaves - 1; ++v) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] += part[v][t][lane];
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (sr.row[i] < 0) continue;
-        float* cp = C + (int64_t)sr.row[i] * ldc + col;
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-          if (col + t < N) atomicAdd(cp + t, acc[t][i]);
-      }
+      for (int t = 0; t < 4; ++t) acc[t] += part[v][t][lane];
     }
-    return;
-  }
-  if ((int)blockIdx.x < huge_blocks + big_blocks) {
-    const int w = p.W_huge + (int)(blockIdx.x - huge_blocks) / nC;
-    const int col = (int)(blockIdx.x - huge_blocks - (w - p.W_huge) * nC) * 64 + 4 * (lane & 15);
-    const int c0 = p.tc_ptr[w], c1 = p.tc_ptr[w + 1];
-    const int share = (c1 - c0 + kBigWaves - 1) / kBigWaves;
-    const int a = min(c0 + wv * share, c1), b = min(a + share, c1);
-    SlotRows sr;
-    if (wv == 0) sr = load_rows(p, w * 16 + (lane >> 4) * 4);
-    f32x4 acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    window_dense<MODE>(p, B, ldb, N, col, vec, a, b, lane, acc);
-    if (wv > 0) {
-#pragma unroll
-      for (int t = 0; t < 4; ++t) part[wv - 1][t][lane] = acc[t];
-    }
-    __syncthreads();
-    if (wv == 0) {
-      for (int v = 0; v < kBigWaves - 1; ++v) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] += part[v][t][lane];
-      }
+    if (!in_huge) {
       store_window(C, ldc, N, col, vec, sr, acc);
+      return;
+    }
+    if (ws) {
+      // partial rows of the part: ws[pt][slot 4g+i][col..col+3]
+      const int g = lane >> 4;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        store_c4(ws + (int64_t)pt * 16 * N, N, 4 * g + i, col, N, vec,
+                 make_float4(acc[0][i], acc[1][i], acc[2][i], acc[3][i]));
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (sr.row[i] < 0) continue;
+      float* cp = C + (int64_t)sr.row[i] * ldc + col;
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (col + t < N) atomicAdd(cp + t, acc[t][i]);
     }
     return;
   }
@@ -652,26 +672,56 @@ __global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p,
   hyb_spmm_window<MODE>(p, B, ldb, N, C, ldc, w, (int)(wave - (int64_t)(w - p.W_big) * nC) * 64, vec);
 }
 
+// Deterministic mode: C rows of huge window w = sum of its parts' partial
+// rows in part order.  One wave per (huge window, 64 columns).
+__global__ __launch_bounds__(256) void hyb_spmm_part_reduce_kernel(SpmmPlanDev p, int N, float* __restrict__ C,
+                                                                   int64_t ldc, int nC, bool vec,
+                                                                   const float* __restrict__ ws) {
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (wave >= (int64_t)p.W_huge * nC) return;
+  const int w = (int)(wave / nC);
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+  const int col = (int)(wave - (int64_t)w * nC) * 64 + 4 * (lane & 15);
+  const SlotRows sr = load_rows(p, w * 16 + g * 4);
+  const int q0 = p.hw_p0[w], q1 = p.hw_p0[w + 1];
+  float4 acc[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int q = q0; q < q1; ++q) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float4 v = load_b4(ws + (int64_t)q * 16 * N, N, 4 * g + i, col, N, vec);
+      acc[i].x += v.x; acc[i].y += v.y; acc[i].z += v.z; acc[i].w += v.w;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (sr.row[i] >= 0) store_c4(C, ldc, sr.row[i], col, N, vec, acc[i]);
+}
+
 // Sparse parts, one row per lane group: first the n_mix slots of dense
-// windows (four per wave and 64 columns, added to the stored dense rows),
-// then the sparse-only windows [W_tc, W) (four waves per window and 64
-// columns).  A kernel of its own so that its small register footprint sets
-// its occupancy (gathers in flight per CU), not the dense path's.
+// windows (four per wave and 64 columns, added to the stored dense rows;
+// n_mix is 0 when the B-stationary kernel adds them itself), then the
+// sparse-only windows [W_tc, W) (four waves per window and 64 columns).  A
+// kernel of its own so that its small register footprint sets its occupancy
+// (gathers in flight per CU), not the dense path's.
 // so_waves: waves for the sparse-only windows (0 when hyb_spmm_sparse_split_kernel
 // stores those rows instead).
 // (256, 7): 71 VGPRs, no spills, 7 waves per SIMD instead of 6; the row
 // gathers are latency-bound (GCN scale-20 SpMM 2.42 -> 2.34 ms).
-__global__ __launch_bounds__(256, 7) void hyb_spmm_sparse_kernel(SpmmPlanDev p, const float* __restrict__ B,
+template <typename TB>
+__global__ __launch_bounds__(256, 7) void hyb_spmm_sparse_kernel(SpmmPlanDev p, const TB* __restrict__ B,
                                                               int64_t ldb, int N, float* __restrict__ C,
-                                                              int64_t ldc, int nC, bool vec, int64_t so_waves) {
+                                                              int64_t ldc, int nC, bool vec, int n_mix,
+                                                              int64_t so_waves) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int64_t mix_waves = (int64_t)((p.n_mix + 3) / 4) * nC;
+  const int64_t mix_waves = (int64_t)((n_mix + 3) / 4) * nC;
   if (wave < mix_waves) {
     const int q = (int)(wave / nC);
     const int col = (int)(wave - (int64_t)q * nC) * 64 + 4 * (lane & 15);
     const int m = q * 4 + (lane >> 4);
-    if (m < p.n_mix) add_slot(p, B, ldb, N, C, ldc, col, vec, p.mix_slot[m]);
+    if (m < n_mix) add_slot(p, B, ldb, N, C, ldc, col, vec, p.mix_slot[m]);
     return;
   }
   const int64_t idx = wave - mix_waves;
@@ -681,23 +731,21 @@ __global__ __launch_bounds__(256, 7) void hyb_spmm_sparse_kernel(SpmmPlanDev p, 
   const int quarter = rest & 3;
   const int n0 = (rest >> 2) * 64;
   const int col = n0 + 4 * (lane & 15);
-  finish_slot(p, B, ldb, N, C, ldc, col, vec, w * 16 + quarter * 4 + (lane >> 4), make_float4(0.f, 0.f, 0.f, 0.f));
+  finish_slot(p, B, ldb, N, C, ldc, col, vec, w * 16 + quarter * 4 + (lane >> 4));
 }
 
 // Sparse-only rows for launches with too few rows to fill the GPU at one row
-// per 16-lane group: one wave per (row, 64 columns), the row's nonzeros
-// dealt to the wave's four lane groups in turn (q - r0 mod 4), eight gathers
-// in flight per group; the four partial float4 are added by xor shuffles and
-// group 0 stores the row.  Rows with more than kHubNnz sparse nonzeros sum
-// with Neumaier compensation, as in finish_slot.
-// One wave's task `wave` = (row wave / nC, column chunk wave % nC).  The
-// row's nonzeros go 64 at a time: lane (g, j) loads nonzero base + 16g + j
-// (column and value, one load each, so a round costs one index latency),
-// lane group g's sixteen are passed to its lanes by shuffles, and the
-// group's 16 B-row gathers are in flight together.  Positions past r1 read
-// the row's last entry with weight 0.  Fixed order: within a group u = 0..15
-// per round, then the xor-shuffle sum over the four groups.
-__device__ __forceinline__ void sparse_split_wave(const SpmmPlanDev& p, const float* __restrict__ B, int64_t ldb,
+// per 16-lane group: one wave per (row, 64 columns).  Task `wave` = (row
+// wave / nC, column chunk wave % nC).  The row's nonzeros go 64 at a time:
+// lane (g, j) loads nonzero base + 16g + j (column and value, one load each,
+// so a round costs one index latency), lane group g's sixteen are passed to
+// its lanes by shuffles, and the group's 16 B-row gathers are in flight
+// together.  Positions past r1 read the row's last entry with weight 0.
+// Fixed order: within a group u = 0..15 per round, then the xor-shuffle sum
+// over the four groups.  Rows with more than kHubNnz sparse nonzeros sum with
+// Neumaier compensation, as in finish_slot.
+template <typename TB>
+__device__ __forceinline__ void sparse_split_wave(const SpmmPlanDev& p, const TB* __restrict__ B, int64_t ldb,
                                                   int N, float* __restrict__ C, int64_t ldc, int nC, bool vec,
                                                   int64_t wave) {
   const int lane = threadIdx.x & 63;
@@ -756,19 +804,34 @@ __device__ __forceinline__ void sparse_split_wave(const SpmmPlanDev& p, const fl
   if (g == 0 && row >= 0) store_c4(C, ldc, row, col, N, vec, s);
 }
 
-__global__ __launch_bounds__(256) void hyb_spmm_sparse_split_kernel(SpmmPlanDev p, const float* __restrict__ B,
+template <typename TB>
+__global__ __launch_bounds__(256) void hyb_spmm_sparse_split_kernel(SpmmPlanDev p, const TB* __restrict__ B,
                                                                     int64_t ldb, int N, float* __restrict__ C,
                                                                     int64_t ldc, int nC, bool vec) {
   sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
 }
 
+// ---------------------------------------------------------------------------
 // B-stationary variant for BF16 plans whose matrices all have K <= kLdsMaxK:
 // a workgroup takes one work unit (a run of dense windows of one matrix,
 // whose columns are B rows [koff, koff + ksz)) and one 64-column chunk,
-// stages B[koff:koff+ksz, n0:n0+64] as BF16 in LDS (one 128-byte row per k,
-// lane j's four columns at byte 8j) and runs the unit's windows against it,
-// a chunk-balanced run of windows per wave.  Each B row then leaves L2 once
-// per workgroup instead of once per window.
+// stages B[koff:koff+ksz, n0:n0+64] as BF16 in LDS and runs the unit's
+// windows against it, a chunk-balanced run of windows per wave.  Each B row
+// then leaves L2 once per workgroup instead of once per window.
+//
+// LDS row r (B row koff + r) starts at byte r * kLdsRowBytes and holds the
+// 64 columns in the order position 16t + 4q + s <-> column n0 + 16q + 4s + t.
+// The MFMA B operand of tile t is read with ds_read_b64_tr_b16: within each
+// 16-lane group, result element e of lane i is element i % 4 of the 8 bytes
+// that lane 4e + i / 4 of the group addresses.  Lane (g, 4e' + q) addresses
+// row tc_cols[c][g][e'] (first read) and tc_cols[c][g][e' + 4] (second read)
+// at position 16t + 4q, so lane j of group g receives B[col(g, e)][n0 + 4j +
+// t] for e = 0..7 -- its MFMA B operand, with no repacking in registers.  The
+// 32 padding bytes per row put the 32 bytes one row contributes to a read at
+// bank offset 40 r + 8 t (mod 64 banks): the 8 rows of a half-wave fall on
+// distinct banks when their row ids differ mod 8, which the plan arranges
+// (k-slot order within each chunk).
+// ---------------------------------------------------------------------------
 
 __device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
   const uint16_t a = __builtin_bit_cast(uint16_t, (__bf16)lo);
@@ -785,49 +848,130 @@ __device__ __forceinline__ uint32_t half_sel(uint32_t x, uint32_t y, int hi) {
 
 constexpr int kLdsThreads = 1024;
 
-__device__ __forceinline__ void lds_chunk_mfma(const uint2* __restrict__ bsh, int j, const int4& ka, const int4& kb,
-                                               const bf16x8& a, f32x4 (&acc)[4]) {
-  uint2 r[8];
-  r[0] = bsh[ka.x * 16 + j];
-  r[1] = bsh[ka.y * 16 + j];
-  r[2] = bsh[ka.z * 16 + j];
-  r[3] = bsh[ka.w * 16 + j];
-  r[4] = bsh[kb.x * 16 + j];
-  r[5] = bsh[kb.y * 16 + j];
-  r[6] = bsh[kb.z * 16 + j];
-  r[7] = bsh[kb.w * 16 + j];
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) unsigned char lds_u8;
+typedef __attribute__((address_space(3))) i16x4 lds_i16x4;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) u32x2 lds_u32x2;
+
+__device__ __forceinline__ i16x4 lds_tr_read(lds_u8* p) {
+  return __builtin_amdgcn_ds_read_tr16_b64_v4i16(reinterpret_cast<lds_i16x4*>(p));
+}
+
+// Stage B[koff : koff + ksz, n0 : n0 + 64] into the LDS rows.  Work item
+// (row r, quarter q) covers columns n0 + 16q .. + 15 and writes, for t =
+// 0..3, the 8 bytes at position 16t + 4q: columns 16q + t, + 4, + 8, + 12.
+// Every thread loads all its items (at most 4: ksz <= 1024) before the
+// first write, so the staging costs one memory latency.  vec16: 16-bit B
+// rows aligned for 16-byte loads and N % 8 == 0; vec: as load_b4.
+template <typename TB>
+__device__ __forceinline__ void stage_b(lds_u8* lds, const TB* __restrict__ B, int64_t ldb, int koff, int ksz,
+                                        int n0, int N, bool vec, bool vec16) {
+  constexpr int U = 4;
+  const int items = ksz * 4;
+  for (int base = threadIdx.x; base < items; base += U * kLdsThreads) {
+    uint32_t w[U][8];  // w[u][2t + h]: columns (8h + t, 8h + 4 + t) of the item as a BF16 pair
+    if constexpr (std::is_same<TB, __bf16>::value) {
+      if (vec16) {
+        uint4 d[U][2];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    uint4 pk;
-    const int hi = t & 1;
-    if (t < 2) {
-      pk.x = half_sel(r[0].x, r[1].x, hi);
-      pk.y = half_sel(r[2].x, r[3].x, hi);
-      pk.z = half_sel(r[4].x, r[5].x, hi);
-      pk.w = half_sel(r[6].x, r[7].x, hi);
-    } else {
-      pk.x = half_sel(r[0].y, r[1].y, hi);
-      pk.y = half_sel(r[2].y, r[3].y, hi);
-      pk.z = half_sel(r[4].y, r[5].y, hi);
-      pk.w = half_sel(r[6].y, r[7].y, hi);
+        for (int u = 0; u < U; ++u) {
+          const int it = min(base + u * kLdsThreads, items - 1);
+          const int c0 = n0 + 16 * (it & 3);
+          const uint4* src = reinterpret_cast<const uint4*>(B + (int64_t)(koff + (it >> 2)) * ldb + c0);
+          d[u][0] = c0 < N ? src[0] : make_uint4(0, 0, 0, 0);
+          d[u][1] = c0 + 8 < N ? src[1] : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t dd[8] = {d[u][0].x, d[u][0].y, d[u][0].z, d[u][0].w,
+                                  d[u][1].x, d[u][1].y, d[u][1].z, d[u][1].w};
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) w[u][2 * t + h] = half_sel(dd[4 * h + t / 2], dd[4 * h + 2 + t / 2], t & 1);
+        }
+        goto write;
+      }
     }
-    const bf16x8 bt = __builtin_bit_cast(bf16x8, pk);
-    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bt, acc[t], 0, 0, 0);
+    {
+      float4 f[U][4];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int it = min(base + u * kLdsThreads, items - 1);
+        const int c0 = n0 + 16 * (it & 3);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) f[u][m] = load_b4(B, ldb, koff + (it >> 2), c0 + 4 * m, N, vec);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) w[u][2 * t + h] = bf16_pair(comp(f[u][2 * h], t), comp(f[u][2 * h + 1], t));
+    }
+  write:
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int it = base + u * kLdsThreads;
+      if (it < items) {
+        lds_u8* row = lds + (it >> 2) * kLdsRowBytes + 8 * (it & 3);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          *reinterpret_cast<lds_u32x2*>(row + 32 * t) = u32x2{w[u][2 * t], w[u][2 * t + 1]};
+      }
+    }
   }
 }
 
-// Column ids (two int4) and A operand (bf16x8) of one chunk for this lane.
-struct ChunkRegs {
-  int4 ka, kb;
+// Column ids of k-slots (g, e') and (g, e' + 4) (the rows this lane
+// addresses in the transposed reads) and the lane's A operand of one chunk.
+struct TrChunk {
+  int k0, k1;
   bf16x8 a;
 };
 
-__device__ __forceinline__ ChunkRegs load_chunk_regs(const SpmmPlanDev& p, int c, int g, int lane) {
-  ChunkRegs r;
-  const int4* cp = reinterpret_cast<const int4*>(p.tc_cols + (int64_t)c * 32 + g * 8);
-  r.ka = cp[0];
-  r.kb = cp[1];
+__device__ __forceinline__ TrChunk load_tr_chunk(const SpmmPlanDev& p, int c, int cidx, int lane) {
+  TrChunk r;
+  const int32_t* cp = p.tc_cols + (int64_t)c * 32 + cidx;
+  r.k0 = cp[0];
+  r.k1 = cp[4];
   r.a = reinterpret_cast<const bf16x8*>(p.tc_vals)[(int64_t)c * 64 + lane];
+  return r;
+}
+
+// cst = 8q - koff * kLdsRowBytes: the lane's position offset within a row,
+// and the unit's first B row
+__device__ __forceinline__ void tr_chunk_mfma(lds_u8* lds, int cst, const TrChunk& r, f32x4 (&acc)[4]) {
+  // 24-bit multiplies (column ids < 2^24): one full-rate v_mad_u32_u24 each
+  lds_u8* p0 = lds + (int)(__umul24((unsigned)r.k0, (unsigned)kLdsRowBytes) + (unsigned)cst);
+  lds_u8* p1 = lds + (int)(__umul24((unsigned)r.k1, (unsigned)kLdsRowBytes) + (unsigned)cst);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const i16x4 lo = lds_tr_read(p0 + 32 * t);
+    const i16x4 hi = lds_tr_read(p1 + 32 * t);
+    const i16x8 b = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.a, __builtin_bit_cast(bf16x8, b), acc[t], 0, 0, 0);
+  }
+}
+
+// Rows of the lane group's 4 window slots and, in plans with mixed windows
+// (kMixed), the bounds of their sparse parts.
+template <bool kMixed>
+struct WinRegs {
+  SlotRows sr;
+  int rp[kMixed ? 5 : 1];
+};
+
+template <bool kMixed>
+__device__ __forceinline__ WinRegs<kMixed> load_win(const SpmmPlanDev& p, int slot0) {
+  WinRegs<kMixed> r;
+  r.sr = load_rows(p, slot0);
+  if constexpr (kMixed) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) r.rp[i] = p.res_ptr[slot0 + i];
+  }
   return r;
 }
 
@@ -836,11 +980,14 @@ __device__ __forceinline__ ChunkRegs load_chunk_regs(const SpmmPlanDev& p, int c
 // the compiler may read it with scalar loads inside the loop that stores C.
 // Returns early per wave (a wave with no windows) or per workgroup (an empty
 // unit); the caller's barrier before the next item's staging keeps the LDS
-// block until every wave is done with it.
+// block until every wave is done with it.  kMixed: windows may have sparse
+// parts, which are added (whole rows, four rows of a lane group at a time)
+// before the window's rows are stored.
+template <typename TB, bool kMixed>
 __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __restrict__ tc_ptr,
-                                         const int4* __restrict__ units, const float* __restrict__ B,
+                                         const int4* __restrict__ units, const TB* __restrict__ B,
                                          int64_t ldb, int N, float* __restrict__ C, int64_t ldc, int nC,
-                                         bool vec, int item, uint2* __restrict__ bsh_all) {
+                                         bool vec, bool vec16, int item, lds_u8* lds) {
   constexpr int kWaves = kLdsThreads / 64;
   const int chunk = item % nC;
   const int4 unit = units[item / nC];
@@ -853,6 +1000,8 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = lane >> 4, j = lane & 15;
   const int col = n0 + 4 * j;
+  const int cidx = g * 8 + ((lane >> 2) & 3);
+  const int cst = 8 * (lane & 3) - koff * kLdsRowBytes;
   // This wave's contiguous run of the unit's windows, balanced by chunk count.
   const int wg_beg = unit.x;
   const int wg_end = unit.y;
@@ -882,52 +1031,51 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   const int last = c_end - 1;
   int w = w_beg;
   int w_next = 0;
-  SlotRows sr{};
-  ChunkRegs r0{}, r1{};
+  WinRegs<kMixed> wr{};
+  TrChunk r0{}, r1{};
   if (has) {
     w_next = tc_ptr[w + 1];
-    sr = load_rows(p, w * 16 + g * 4);
-    r0 = load_chunk_regs(p, c_beg, g, lane);
-    r1 = load_chunk_regs(p, min(c_beg + 1, last), g, lane);
+    wr = load_win<kMixed>(p, w * 16 + g * 4);
+    r0 = load_tr_chunk(p, c_beg, cidx, lane);
+    r1 = load_tr_chunk(p, min(c_beg + 1, last), cidx, lane);
   }
-  // Stage B[koff:koff+ksz, n0:n0+64] as BF16: eight independent float4 loads
-  // per thread in flight before the packed stores.
-  for (int base = threadIdx.x; base < ksz * 16; base += 8 * kLdsThreads) {
-    float4 v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = base + u * kLdsThreads;
-      v[u] = i < ksz * 16 ? load_b4(B, ldb, koff + (i >> 4), n0 + 4 * (i & 15), N, vec)
-                          : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = base + u * kLdsThreads;
-      if (i < ksz * 16) bsh_all[i] = make_uint2(bf16_pair(v[u].x, v[u].y), bf16_pair(v[u].z, v[u].w));
-    }
-  }
+  stage_b(lds, B, ldb, koff, ksz, n0, N, vec, vec16);
   __syncthreads();
   if (!has) return;
-  // the unit's column ids k lie in [koff, koff + ksz): LDS row k - koff
-  const uint2* bsh = bsh_all - (int64_t)koff * 16;
   f32x4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  ChunkRegs r2;
+  auto finish = [&](const WinRegs<kMixed>& cur) {
+    if constexpr (kMixed) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int a = cur.rp[i], b = cur.rp[i + 1];
+        if (a < b) {
+          const float4 s = sparse_row_sum<4>(p, B, ldb, N, col, vec, a, b, b - a > kHubNnz);
+          acc[0][i] += s.x;
+          acc[1][i] += s.y;
+          acc[2][i] += s.z;
+          acc[3][i] += s.w;
+        }
+      }
+    }
+    store_window<true>(C, ldc, N, col, vec, cur.sr, acc);
+  };
+  TrChunk r2;
   int c = c_beg;
-  auto step = [&](const ChunkRegs& use, ChunkRegs& fill) {
-    fill = load_chunk_regs(p, min(c + 2, last), g, lane);
+  auto step = [&](const TrChunk& use, TrChunk& fill) {
+    fill = load_tr_chunk(p, min(c + 2, last), cidx, lane);
     if (c >= w_next) {  // window w is complete: store it
-      const SlotRows sr_next = load_rows(p, (w + 1) * 16 + g * 4);
+      const WinRegs<kMixed> wn = load_win<kMixed>(p, (w + 1) * 16 + g * 4);
       const int w_next2 = tc_ptr[w + 2];
-      store_window<true>(C, ldc, N, col, vec, sr, acc);
+      finish(wr);
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-      sr = sr_next;
+      wr = wn;
       ++w;
       w_next = w_next2;
     }
-    lds_chunk_mfma(bsh, j, use.ka, use.kb, use.a, acc);
+    tr_chunk_mfma(lds, cst, use, acc);
     ++c;
   };
   while (c + 3 <= c_end) {
@@ -937,7 +1085,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   }
   if (c < c_end) step(r0, r2);
   if (c < c_end) step(r1, r0);
-  store_window<true>(C, ldc, N, col, vec, sr, acc);
+  finish(wr);
 }
 
 // units: the work-unit list of this launch.  kLoop = false: one workgroup
@@ -955,19 +1103,21 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
 // 16i + 15 of tail_tasks.  Under the XCD grouping those are the last slot of
 // each XCD, which holds the fewest items when a range does not divide evenly;
 // one-item launches add tail blocks past the items when CUs are free.
-template <bool kLoop>
+template <bool kLoop, typename TB, bool kMixed>
 __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p,
                                                                         const int32_t* __restrict__ tc_ptr,
                                                                         const int4* __restrict__ units,
-                                                                        const float* __restrict__ B,
+                                                                        const TB* __restrict__ B,
                                                                         int64_t ldb, int N, float* __restrict__ C,
                                                                         int64_t ldc, int nC, int n_items, bool vec,
-                                                                        LdsXcdRanges xr, bool xcd, int tail_blocks,
-                                                                        int tail_tasks) {
-  extern __shared__ uint2 bsh_all[];  // [ksz][16]: 4 BF16 per entry
+                                                                        bool vec16, LdsXcdRanges xr, bool xcd,
+                                                                        int tail_blocks, int tail_tasks) {
+  extern __shared__ uint4 lds_dyn[];  // [ksz][kLdsRowBytes]
+  lds_u8* lds = (lds_u8*)lds_dyn;
   const int tb = (int)gridDim.x - 1 - (int)blockIdx.x;
   if (!kLoop) {
-    if ((int)blockIdx.x < n_items) lds_item(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, blockIdx.x, bsh_all);
+    if ((int)blockIdx.x < n_items)
+      lds_item<TB, kMixed>(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, vec16, blockIdx.x, lds);
     if (tb < tail_blocks) {
       const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
       if (task < tail_tasks) sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, task);
@@ -990,7 +1140,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   }
   for (int item = beg; item < end; item += step) {
     __syncthreads();  // every wave is done reading the previous item's block
-    lds_item(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, item, bsh_all);
+    lds_item<TB, kMixed>(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, vec16, item, lds);
   }
   if (tb < tail_blocks) {
     const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
@@ -998,17 +1148,20 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   }
 }
 
-// Remaining pieces of long sparse rows: lane group g of the wave takes
-// nonzeros q = beg + g, beg + g + 4, ...; the four partial float4 are reduced
-// across groups and added to C with atomicAdd (C already holds the window
-// pass's value of the row).
-__global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, const float* __restrict__ B,
+// Remaining pieces of long sparse rows, pieces [pc0, pc1): lane group g of
+// the wave takes nonzeros q = beg + g, beg + g + 4, ...; the four partial
+// float4 are reduced across groups and added to C with atomicAdd (C already
+// holds the window pass's value of the row), or, with ws (deterministic
+// mode), stored to ws[piece][N] for hyb_spmm_piece_reduce_kernel.
+template <typename TB>
+__global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, const TB* __restrict__ B,
                                                              int64_t ldb, int N, float* __restrict__ C,
-                                                             int64_t ldc, int nC, bool vec) {
+                                                             int64_t ldc, int nC, bool vec, int pc0, int pc1,
+                                                             float* __restrict__ ws) {
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int pc = wave / nC;
-  if (pc >= p.npieces) return;
-  const int n0 = (wave - pc * nC) * 64;
+  const int pc = pc0 + wave / nC;
+  if (pc >= pc1) return;
+  const int n0 = (wave - (pc - pc0) * nC) * 64;
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
   const int col = n0 + 4 * (lane & 15);
@@ -1045,20 +1198,54 @@ __global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, cons
     s.z += __shfl_xor(s.z, off);
     s.w += __shfl_xor(s.w, off);
   }
-  if (g == 0) {
-    float* cp = C + (int64_t)p.piece_row[pc] * ldc + col;
-    if (col < N) atomicAdd(cp, s.x);
-    if (col + 1 < N) atomicAdd(cp + 1, s.y);
-    if (col + 2 < N) atomicAdd(cp + 2, s.z);
-    if (col + 3 < N) atomicAdd(cp + 3, s.w);
+  if (g != 0) return;
+  if (ws) {
+    store_c4(ws, N, pc, col, N, vec, s);
+    return;
   }
+  float* cp = C + (int64_t)p.piece_row[pc] * ldc + col;
+  if (col < N) atomicAdd(cp, s.x);
+  if (col + 1 < N) atomicAdd(cp + 1, s.y);
+  if (col + 2 < N) atomicAdd(cp + 2, s.z);
+  if (col + 3 < N) atomicAdd(cp + 3, s.w);
+}
+
+// Deterministic mode: C[row] += the row's pieces in piece order, for split
+// rows [sr0, sr1).  One 16-lane group per (split row, 64 columns); C[row] is
+// read and written by that group alone.
+__global__ __launch_bounds__(256) void hyb_spmm_piece_reduce_kernel(SpmmPlanDev p, int N, float* __restrict__ C,
+                                                                    int64_t ldc, int nC, bool vec, int sr0,
+                                                                    int sr1, const float* __restrict__ ws) {
+  const int64_t task = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int r = sr0 + (int)(task / nC);
+  if (r >= sr1) return;
+  const int col = (int)(task - (int64_t)(r - sr0) * nC) * 64 + 4 * (threadIdx.x & 15);
+  const int row = p.srow_row[r];
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int q = p.srow_p0[r]; q < p.srow_p1[r]; ++q) {
+    const float4 v = load_b4(ws, N, q, col, N, vec);
+    s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+  }
+  const float4 c = load_b4(C, ldc, row, col, N, vec);
+  store_c4(C, ldc, row, col, N, vec, make_float4(c.x + s.x, c.y + s.y, c.z + s.z, c.w + s.w));
+}
+
+template <bool kLoop, typename TB>
+static void set_lds_bytes(int bytes) {
+  for (const void* f : {reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<kLoop, TB, false>),
+                        reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<kLoop, TB, true>)})
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
 void prepare_spmm_kernels(const SpmmPlanDev& p) {
-  if (p.lds_units && p.lds_max_k * 128 > 65536)
-    for (const void* f : {reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<false>),
-                          reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<true>)})
-      (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, p.lds_max_k * 128);
+  const int bytes = p.lds_max_k * kLdsRowBytes;
+  if (!p.lds_units || bytes <= 65536) return;
+  set_lds_bytes<false, float>(bytes);
+  set_lds_bytes<true, float>(bytes);
+  set_lds_bytes<false, __bf16>(bytes);
+  set_lds_bytes<true, __bf16>(bytes);
+  set_lds_bytes<false, _Float16>(bytes);
+  set_lds_bytes<true, _Float16>(bytes);
 }
 
 int device_cus() {
@@ -1108,12 +1295,22 @@ static int lds_list_override() {
   return t;
 }
 
-hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, int N, float* C, int64_t ldc,
-                           hipStream_t s) {
-  if (p.W == 0 || N == 0) return hipSuccess;
+int64_t spmm_workspace_floats(const SpmmPlanDev& p, int N) {
+  return ((int64_t)p.npieces + (int64_t)p.n_parts * 16) * N;
+}
+
+template <typename TB>
+static hipError_t launch_hyb_spmm_t(const SpmmPlanDev& p, const TB* B, int64_t ldb, int N, float* C, int64_t ldc,
+                                    bool det, float* ws, hipStream_t s) {
+  const int eb = (int)sizeof(TB);
   const bool vec = (N % 4 == 0) && (ldb % 4 == 0) && (ldc % 4 == 0) &&
-                   ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && ((reinterpret_cast<uintptr_t>(C) & 15) == 0);
+                   ((reinterpret_cast<uintptr_t>(B) & (4 * eb - 1)) == 0) &&
+                   ((reinterpret_cast<uintptr_t>(C) & 15) == 0);
+  const bool vec16 = eb == 2 && vec && N % 8 == 0 && ldb % 8 == 0 && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
   const int nC = (N + 63) / 64;
+  // workspace regions of the deterministic mode: pieces, then huge-window parts
+  float* ws_pieces = det ? ws : nullptr;
+  float* ws_parts = det ? ws + (int64_t)p.npieces * N : nullptr;
   // B-stationary kernel for every BF16 plan whose matrices all have K <=
   // kLdsMaxK (measured on the block-pruned layer: 9.0 us vs 15.0 us
   // streaming at N = 128, 20.8 vs 82 us at N = 4096); otherwise the
@@ -1128,8 +1325,9 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
   const int cus = device_cus();
   const bool fold = lds && so_rows > 0 && so_rows * nC <= kLdsTailMaxTasks &&
                     (so_rows * nC + kLdsThreads / 64 - 1) / (kLdsThreads / 64) <= cus / kXcds * kXcds;
-  // The LDS kernel runs every dense window; otherwise the streaming kernel
-  // does (its first blocks the big windows).  The sparse-only windows follow.
+  // The LDS kernel runs every dense window with its sparse part; otherwise
+  // the streaming kernel does the dense parts (its first blocks the big
+  // windows) and the sparse kernel the sparse parts.
   if (lds) {
     int t = lds_list_override();
     if (t < 0 && nC <= kLdsPickMax) t = p.lds_pick[nC];
@@ -1157,18 +1355,23 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
     // are free, else the last item blocks take the tail
     if (items <= cus && items + tail_blocks <= cus) grid = (int)items + tail_blocks;
     if (items <= cus) grid = std::max(grid, tail_blocks);
-    if (items <= cus)
-      hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel<false>, dim3(grid), dim3(kLdsThreads), (size_t)p.lds_max_k * 128,
-                         s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C, ldc, nC, (int)items, vec, xr,
-                         false, tail_blocks, tail_tasks);
-    else
-      hipLaunchKernelGGL(hyb_spmm_lds_bf16_kernel<true>, dim3(grid), dim3(kLdsThreads), (size_t)p.lds_max_k * 128,
-                         s, p, p.tc_ptr, p.lds_units + p.lds_off[t], B, ldb, N, C, ldc, nC, (int)items, vec, xr,
-                         xcd, tail_blocks, tail_tasks);
+    const size_t lds_bytes = (size_t)p.lds_max_k * kLdsRowBytes;
+    const int4* units = p.lds_units + p.lds_off[t];
+    const bool mixed = p.n_mix > 0;
+#define STILE_LDS_LAUNCH(LOOP, MIX)                                                                          \
+  hipLaunchKernelGGL((hyb_spmm_lds_bf16_kernel<LOOP, TB, MIX>), dim3(grid), dim3(kLdsThreads), lds_bytes, s, p, \
+                     p.tc_ptr, units, B, ldb, N, C, ldc, nC, (int)items, vec, vec16, xr, LOOP && xcd,          \
+                     tail_blocks, tail_tasks)
+    if (items <= cus) {
+      if (mixed) STILE_LDS_LAUNCH(false, true); else STILE_LDS_LAUNCH(false, false);
+    } else {
+      if (mixed) STILE_LDS_LAUNCH(true, true); else STILE_LDS_LAUNCH(true, false);
+    }
+#undef STILE_LDS_LAUNCH
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   } else if (p.W_tc > 0) {
-    if (p.n_parts > 0) {
+    if (p.n_parts > 0 && !det) {
       hipError_t e = launch_zero_window_rows(p, p.W_huge, C, ldc, N, s);
       if (e != hipSuccess) return e;
     }
@@ -1180,27 +1383,39 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
     const int bb = (int)big_blocks;
     switch (p.blk_dtype) {
       case 1:
-        hipLaunchKernelGGL(hyb_spmm_kernel<1>, grid, block, 0, s, p, B, ldb, N, C, ldc, nC, bb, vec);
+        hipLaunchKernelGGL((hyb_spmm_kernel<1, TB>), grid, block, 0, s, p, B, ldb, N, C, ldc, nC, bb, vec,
+                           ws_parts);
         break;
       case 2:
-        hipLaunchKernelGGL(hyb_spmm_kernel<2>, grid, block, 0, s, p, B, ldb, N, C, ldc, nC, bb, vec);
+        hipLaunchKernelGGL((hyb_spmm_kernel<2, TB>), grid, block, 0, s, p, B, ldb, N, C, ldc, nC, bb, vec,
+                           ws_parts);
         break;
       default:
-        hipLaunchKernelGGL(hyb_spmm_kernel<0>, grid, block, 0, s, p, B, ldb, N, C, ldc, nC, bb, vec);
+        hipLaunchKernelGGL((hyb_spmm_kernel<0, TB>), grid, block, 0, s, p, B, ldb, N, C, ldc, nC, bb, vec,
+                           ws_parts);
         break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (det && p.W_huge > 0) {
+      const int64_t waves = (int64_t)p.W_huge * nC;
+      hipLaunchKernelGGL(hyb_spmm_part_reduce_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, p, N, C,
+                         ldc, nC, vec, ws_parts);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
   }
   // Sparse-only rows (unless folded) one per lane group, or, when that
   // leaves fewer than 16 waves per CU for rows of 8 or more nonzeros on
-  // average, one per wave.
+  // average, one per wave; before them, unless the LDS kernel added them,
+  // the sparse parts of the dense windows' rows.
+  const int n_mix = lds ? 0 : p.n_mix;
   const int64_t so_left = fold ? 0 : so_rows;
   const bool split = so_left > 0 && so_left * nC / 4 < 4096 && p.nnz_sparse_only >= 8 * so_left;
-  const int64_t sparse_waves = (int64_t)((p.n_mix + 3) / 4) * nC + (split ? 0 : so_left * nC / 4);
+  const int64_t sparse_waves = (int64_t)((n_mix + 3) / 4) * nC + (split ? 0 : so_left * nC / 4);
   if (split) {
     const int64_t blocks = (so_rows * nC + 3) / 4;
-    hipLaunchKernelGGL(hyb_spmm_sparse_split_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, B, ldb, N, C,
+    hipLaunchKernelGGL(hyb_spmm_sparse_split_kernel<TB>, dim3((unsigned)blocks), dim3(256), 0, s, p, B, ldb, N, C,
                        ldc, nC, vec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1208,18 +1423,45 @@ hipError_t launch_hyb_spmm(const SpmmPlanDev& p, const float* B, int64_t ldb, in
   if (sparse_waves > 0) {
     const int64_t blocks = (sparse_waves + 3) / 4;
     if (blocks > INT32_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(hyb_spmm_sparse_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, B, ldb, N, C, ldc, nC,
-                       vec, split ? (int64_t)0 : so_left * nC / 4);
+    hipLaunchKernelGGL(hyb_spmm_sparse_kernel<TB>, dim3((unsigned)blocks), dim3(256), 0, s, p, B, ldb, N, C, ldc,
+                       nC, vec, n_mix, split ? (int64_t)0 : so_left * nC / 4);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  if (p.npieces > 0) {
-    const int64_t fw = (int64_t)p.npieces * nC;
-    hipLaunchKernelGGL(hyb_spmm_fixup_kernel, dim3((unsigned)((fw + 3) / 4)), dim3(256), 0, s, p, B, ldb, N, C,
-                       ldc, nC, vec);
-    return hipGetLastError();
+  // Pieces of long rows.  The LDS kernel summed its windows' rows whole, so
+  // only the sparse-only rows' pieces remain after it.
+  const int pc0 = lds ? p.npieces_dense : 0;
+  if (p.npieces > pc0) {
+    const int64_t fw = (int64_t)(p.npieces - pc0) * nC;
+    hipLaunchKernelGGL(hyb_spmm_fixup_kernel<TB>, dim3((unsigned)((fw + 3) / 4)), dim3(256), 0, s, p, B, ldb, N,
+                       C, ldc, nC, vec, pc0, p.npieces, ws_pieces);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (det) {
+      const int sr0 = lds ? p.nsrows_dense : 0;
+      const int64_t tasks = (int64_t)(p.nsrows - sr0) * nC;
+      if (tasks > 0) {
+        hipLaunchKernelGGL(hyb_spmm_piece_reduce_kernel, dim3((unsigned)((tasks + 15) / 16)), dim3(256), 0, s, p,
+                           N, C, ldc, nC, vec, sr0, p.nsrows, ws_pieces);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+      }
+    }
   }
   return hipSuccess;
+}
+
+hipError_t launch_hyb_spmm(const SpmmPlanDev& p, int b_dtype, const void* B, int64_t ldb, int N, float* C,
+                           int64_t ldc, bool deterministic, float* ws, hipStream_t s) {
+  if (p.W == 0 || N == 0) return hipSuccess;
+  switch (b_dtype) {
+    case 1:
+      return launch_hyb_spmm_t(p, static_cast<const __bf16*>(B), ldb, N, C, ldc, deterministic, ws, s);
+    case 2:
+      return launch_hyb_spmm_t(p, static_cast<const _Float16*>(B), ldb, N, C, ldc, deterministic, ws, s);
+    default:
+      return launch_hyb_spmm_t(p, static_cast<const float*>(B), ldb, N, C, ldc, deterministic, ws, s);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -481,6 +481,45 @@ void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, in
     }
 }
 
+// k-slot order within each chunk of 32 dense columns of a window, for the
+// B-stationary kernel: k-slot ks = 4e + g is read by lane group g, and one
+// transposed LDS read of a half-wave gathers the B rows of the 8 k-slots
+// {16h + 4e' + g : e' = 0..3, g in {2P, 2P + 1}} (h, P = 0, 1).  Those rows
+// fall on distinct LDS banks when they differ mod 8 (rows of kLdsRowBytes
+// bytes).  The columns are dealt to the four k-slot sets by residue class
+// (col - koff) mod 8, so every set holds distinct residues whenever no class
+// has more than 4 of the chunk's columns (two 16-column dense blocks: 2
+// each).  Only the order changes; the last chunk keeps its real columns in
+// its first n k-slots.
+void order_chunk_slots(std::vector<DenseCol>& cols, int64_t koff) {
+  for (size_t base = 0; base < cols.size(); base += 32) {
+    const int n = (int)std::min<size_t>(32, cols.size() - base);
+    // the real k-slots of each set
+    std::vector<int> slots[4];
+    for (int ks = 0; ks < n; ++ks) slots[(ks >> 4) * 2 + ((ks & 3) >> 1)].push_back(ks);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    auto res = [&](int i) { return (int)((cols[base + i].col - koff) & 7); };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return res(a) < res(b); });
+    unsigned used[4] = {0, 0, 0, 0};  // residues present per set
+    size_t fill[4] = {0, 0, 0, 0};
+    std::vector<DenseCol> out(n);
+    for (int i = 0; i < n; ++i) {
+      const int r = res(order[i]);
+      int pick = -1;
+      for (int d = 0; d < 4 && pick < 0; ++d) {  // preferred set first, then the others
+        const int set = (i + d) & 3;
+        if (fill[set] < slots[set].size() && !(used[set] >> r & 1u)) pick = set;
+      }
+      for (int set = 0; set < 4 && pick < 0; ++set)
+        if (fill[set] < slots[set].size()) pick = set;
+      used[pick] |= 1u << r;
+      out[slots[pick][fill[pick]++]] = cols[base + order[i]];
+    }
+    std::copy(out.begin(), out.end(), cols.begin() + base);
+  }
+}
+
 int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const int32_t W = (int32_t)(bd.win_rows.size() / 16);
 
@@ -589,20 +628,32 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
     if (cnt[sl + 1] > cnt[sl]) mix_slot.push_back((int32_t)sl);
   // Long rows: the window pass takes the first kSplitNnz nonzeros of a slot,
   // the rest is cut into pieces of kSplitNnz for the fix-up kernel.
+  // Pieces are in slot order, so those of dense windows' slots come first
+  // (npieces_dense); each split row's pieces are consecutive (srow_*).
   int32_t hub_rows = 0;
-  std::vector<int32_t> piece_row, piece_beg, piece_end;
+  std::vector<int32_t> piece_row, piece_beg, piece_end, srow_row, srow_p0, srow_p1;
+  int32_t npieces_dense = 0, nsrows_dense = 0;
   for (size_t sl = 0; sl + 1 < cnt.size(); ++sl) {
     const int64_t a = cnt[sl], b = cnt[sl + 1];
     if (b - a > stile::kHubNnz) hub_rows++;
     if (b - a <= stile::kSplitNnz) continue;
     const int32_t row = bd.win_rows[sl];
+    srow_row.push_back(row);
+    srow_p0.push_back((int32_t)piece_row.size());
     for (int64_t q = a + stile::kSplitNnz; q < b; q += stile::kSplitNnz) {
       piece_row.push_back(row);
       piece_beg.push_back((int32_t)q);
       piece_end.push_back((int32_t)std::min<int64_t>(b, q + stile::kSplitNnz));
     }
+    srow_p1.push_back((int32_t)piece_row.size());
+    if ((int64_t)sl < (int64_t)W_tc * 16) {
+      npieces_dense = (int32_t)piece_row.size();
+      nsrows_dense = (int32_t)srow_row.size();
+    }
   }
 
+  if (lds_ok)
+    for (int32_t w = 0; w < W_tc; ++w) order_chunk_slots(bd.win_cols[w], bd.inst_cols[window_matrix(bd, w)]);
   // Dense chunks: cut each window's columns into chunks of 32 k-slots (the
   // last one padded with value-0 slots that point at the chunk's first column).
   std::vector<int32_t> tc_ptr(W + 1, 0);
@@ -621,13 +672,16 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
     build_lds_units(bd, tc_ptr, W_tc, stile::device_cus(), lds_units, lds_off, lds_cnt, lds_max_k, lds_pick,
                     lds_xcd);
   // Parts of the huge windows: consecutive ranges of at most kHugeChunks chunks.
-  std::vector<int32_t> part_win, part_c0, part_c1;
-  for (int32_t w = 0; w < W_huge; ++w)
+  std::vector<int32_t> part_win, part_c0, part_c1, hw_p0;
+  for (int32_t w = 0; w < W_huge; ++w) {
+    hw_p0.push_back((int32_t)part_win.size());
     for (int64_t c = tc_ptr[w]; c < tc_ptr[w + 1]; c += stile::kHugeChunks) {
       part_win.push_back(w);
       part_c0.push_back((int32_t)c);
       part_c1.push_back((int32_t)std::min<int64_t>(c + stile::kHugeChunks, tc_ptr[w + 1]));
     }
+  }
+  hw_p0.push_back((int32_t)part_win.size());
   const size_t val_bytes = blk_dtype == STILE_BLK_F32 ? 4 : 2;
   std::vector<int32_t> tc_cols((size_t)nchunks * 32);
   std::vector<unsigned char> tc_vals((size_t)nchunks * 64 * 8 * val_bytes);
@@ -705,6 +759,10 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const size_t o_pc0 = lay.add(part_c0.size() * 4);
   const size_t o_pc1 = lay.add(part_c1.size() * 4);
   const size_t o_units = lay.add(lds_units.size() * 4);
+  const size_t o_srow = lay.add(srow_row.size() * 4);
+  const size_t o_sp0 = lay.add(srow_p0.size() * 4);
+  const size_t o_sp1 = lay.add(srow_p1.size() * 4);
+  const size_t o_hwp0 = lay.add(hw_p0.size() * 4);
   HostBlob blob;
   blob.data.assign(lay.bytes, 0);
   blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
@@ -726,6 +784,10 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   blob.put(o_pc0, part_c0.data(), part_c0.size() * 4);
   blob.put(o_pc1, part_c1.data(), part_c1.size() * 4);
   blob.put(o_units, lds_units.data(), lds_units.size() * 4);
+  blob.put(o_srow, srow_row.data(), srow_row.size() * 4);
+  blob.put(o_sp0, srow_p0.data(), srow_p0.size() * 4);
+  blob.put(o_sp1, srow_p1.data(), srow_p1.size() * 4);
+  blob.put(o_hwp0, hw_p0.data(), hw_p0.size() * 4);
 
   StilePlan* plan = new (std::nothrow) StilePlan();
   if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
@@ -762,6 +824,13 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.lds_xcd_host = plan->lds_xcd.empty() ? nullptr : plan->lds_xcd.data();
   d.nchunks = (int32_t)nchunks;
   d.npieces = (int32_t)piece_row.size();
+  d.npieces_dense = npieces_dense;
+  d.nsrows = (int32_t)srow_row.size();
+  d.nsrows_dense = nsrows_dense;
+  d.srow_row = dptr<int32_t>(plan, o_srow);
+  d.srow_p0 = dptr<int32_t>(plan, o_sp0);
+  d.srow_p1 = dptr<int32_t>(plan, o_sp1);
+  d.hw_p0 = dptr<int32_t>(plan, o_hwp0);
   d.piece_row = dptr<int32_t>(plan, o_prow);
   d.piece_beg = dptr<int32_t>(plan, o_pbeg);
   d.piece_end = dptr<int32_t>(plan, o_pend);
@@ -1089,15 +1158,39 @@ int stile_plan_create_sddmm_batched(const StileSddmmPack* packs, int32_t count, 
   return create_sddmm(packs, count, blk_dtype, out);
 }
 
-int stile_spmm_run(const StilePlan* plan, const float* B, int64_t ldb, int32_t N, float* C, int64_t ldc,
-                   void* stream) {
+int stile_spmm_workspace_bytes(const StilePlan* plan, int32_t N, int32_t flags, int64_t* bytes) {
   if (!plan || plan->kind != 1) return fail(STILE_E_INVALID, "not an SpMM plan");
+  if (!bytes) return fail(STILE_E_INVALID, "null output pointer");
+  if (N < 0) return fail(STILE_E_INVALID, "negative N");
+  if (flags & ~STILE_RUN_DETERMINISTIC) return fail(STILE_E_INVALID, "unknown run flags");
+  *bytes = (flags & STILE_RUN_DETERMINISTIC) ? stile::spmm_workspace_floats(plan->spmm, N) * 4 : 0;
+  return STILE_OK;
+}
+
+int stile_spmm_run_ex(const StilePlan* plan, int32_t b_dtype, const void* B, int64_t ldb, int32_t N, float* C,
+                      int64_t ldc, int32_t flags, void* workspace, int64_t workspace_bytes, void* stream) {
+  if (!plan || plan->kind != 1) return fail(STILE_E_INVALID, "not an SpMM plan");
+  if (b_dtype < STILE_BLK_F32 || b_dtype > STILE_BLK_F16) return fail(STILE_E_INVALID, "unknown b_dtype");
+  if (flags & ~STILE_RUN_DETERMINISTIC) return fail(STILE_E_INVALID, "unknown run flags");
   if (N < 0) return fail(STILE_E_INVALID, "negative N");
   if (N == 0 || plan->spmm.W == 0) return STILE_OK;
   if (!C || (plan->spmm.K > 0 && !B)) return fail(STILE_E_INVALID, "null pointer");
   if (ldb < N || ldc < N) return fail(STILE_E_INVALID, "leading dimension smaller than N");
-  hipError_t e = stile::launch_hyb_spmm(plan->spmm, B, ldb, N, C, ldc, (hipStream_t)stream);
+  const bool det = (flags & STILE_RUN_DETERMINISTIC) != 0;
+  const int64_t need = det ? stile::spmm_workspace_floats(plan->spmm, N) * 4 : 0;
+  if (need > 0) {
+    if (!workspace || workspace_bytes < need)
+      return fail(STILE_E_INVALID, "workspace smaller than stile_spmm_workspace_bytes");
+    if (reinterpret_cast<uintptr_t>(workspace) & 15) return fail(STILE_E_INVALID, "workspace not 16-byte aligned");
+  }
+  hipError_t e = stile::launch_hyb_spmm(plan->spmm, b_dtype, B, ldb, N, C, ldc, det,
+                                        static_cast<float*>(workspace), (hipStream_t)stream);
   return e == hipSuccess ? STILE_OK : hip_fail(e, "hyb_spmm launch");
+}
+
+int stile_spmm_run(const StilePlan* plan, const float* B, int64_t ldb, int32_t N, float* C, int64_t ldc,
+                   void* stream) {
+  return stile_spmm_run_ex(plan, STILE_BLK_F32, B, ldb, N, C, ldc, 0, nullptr, 0, stream);
 }
 
 int stile_sddmm_run(const StilePlan* plan, const float* X, int64_t ldx, const float* Y, int64_t ldy, int32_t Kf,
