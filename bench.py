@@ -247,6 +247,10 @@ def _gf(flops, ms):
 # ---------------------------------------------------------------------------
 
 def run_pruned_layer(args, dev):
+    """Config 3: the block-pruned layer A 3072 x 768 times B 768 x N with B in
+    BF16 (the reference keeps B in its run dtype) and C FP32.  Secondary
+    figures on the same plan: FP32 B, N = 128, and (unless --no-batch) the
+    120-matrix encoder batch at N = 128 in one launch."""
     import numpy as np
     import torch
     from oracle import ops
@@ -255,27 +259,32 @@ def run_pruned_layer(args, dev):
     A, B = datasets.pruned_layer(seed=0, N=N)
     M, K = A.shape
     t0 = time.perf_counter()
-    plan, _, tuning = api.tune_plan("spmm", A, "bfloat16", N, log=progress,
+    plan, _, tuning = api.tune_plan("spmm", A, "bfloat16", N, log=progress, b_dtype="bfloat16",
                                     cache=cache_path(args, f"pruned_layer_n{N}"))
     t_plan = time.perf_counter() - t0
     info = plan.info()
-    Bd = torch.from_numpy(B).to(dev)
+    Bf = torch.from_numpy(B).to(dev)
+    Bd = Bf.to(torch.bfloat16)
     C = torch.empty((M, N), device=dev, dtype=torch.float32)
     ms = time_launches(lambda: plan.run_spmm(Bd, C), args.steps, args.warmup)
     C_h = C.cpu().numpy()
-    ms_csr, ms_vendor, ms_hyb = _spmm_comparators(A, Bd, max(args.steps // 4, 10), 5, hyb=True)
+    ms_f32 = time_launches(lambda: plan.run_spmm(Bf, C), args.steps, args.warmup)
+    ms_csr, ms_vendor, ms_hyb = _spmm_comparators(A, Bf, max(args.steps // 4, 10), 5, hyb=True)
     flops = 2 * A.nnz * N
     nblocks = full_blocks(A)
+    alg = spmm_alg_bytes_blocks(nblocks, A.nnz, M, K, N, s_b=2)
     n128 = None
     if not args.no_n128:
-        B128 = torch.from_numpy(np.ascontiguousarray(B[:, :128])).to(dev)
+        B128f = torch.from_numpy(np.ascontiguousarray(B[:, :128])).to(dev)
+        B128 = B128f.to(torch.bfloat16)
         C128 = torch.empty((M, 128), device=dev, dtype=torch.float32)
         ms128 = time_launches(lambda: plan.run_spmm(B128, C128), args.steps, args.warmup)
-        ms128_csr, ms128_vendor = _spmm_comparators(A, B128, args.steps, args.warmup)
+        ms128_csr, ms128_vendor = _spmm_comparators(A, B128f, args.steps, args.warmup)
         n128 = {"hybrid_us": round(ms128 * 1e3, 2), "csr_us": round(ms128_csr * 1e3, 2),
                 "vendor_us": None if ms128_vendor is None else round(ms128_vendor * 1e3, 2),
-                "hybrid_gflops": _gf(2 * A.nnz * 128, ms128),
-                "hbm_frac": roofline(spmm_alg_bytes_blocks(nblocks, A.nnz, M, K, 128), ms128)["frac"]}
+                "hybrid_over_csr": round(ms128_csr / ms128, 3), "hybrid_gflops": _gf(2 * A.nnz * 128, ms128),
+                "hbm_frac": roofline(spmm_alg_bytes_blocks(nblocks, A.nnz, M, K, 128, s_b=2), ms128)["frac"],
+                "b_dtype": "bf16"}
     # back-to-back launch floor (a one-element kernel): what a launch costs
     # before any work; it bounds small-N launches
     tiny = torch.zeros(1, device=dev)
@@ -285,34 +294,47 @@ def run_pruned_layer(args, dev):
     cp_dst = torch.empty_like(cp_src)
     ms_copy = time_launches(lambda: cp_dst.copy_(cp_src), 20, 3)
     copy_gbs = 2 * cp_src.numel() * 4 / (ms_copy * 1e-3) / 1e9
+    del cp_src, cp_dst
     traffic = measured_traffic("hyb_spmm_traffic_pruned_layer.json") if N == 4096 else None
     out = line("SpMM GFLOP/s (2*nnz*N), hybrid fused vs plain CSR", flops / (ms * 1e-3) / 1e9, "GFLOP/s",
-               ms, args, "bf16 MFMA blocks + f32 sparse, f32 accumulate",
+               ms, args, "bf16 B and MFMA blocks, f32 sparse products, f32 accumulate and C",
                {"workload": "hybrid SpMM, block-pruned layer A 3072x768 (10% of 16x16 blocks + 1% residual, "
-                            f"nnz={A.nnz}) x B 768x{N} FP32", "M": M, "K": K, "N": N, "nnz": int(A.nnz),
-                "global_batch": 1, "parallelism": "single GPU"},
-               roofline=roofline(spmm_alg_bytes_blocks(nblocks, A.nnz, M, K, N), ms, traffic),
+                            f"nnz={A.nnz}) x B 768x{N} BF16 -> C FP32", "M": M, "K": K, "N": N,
+                "nnz": int(A.nnz), "b_dtype": "bf16", "global_batch": 1, "parallelism": "single GPU"},
+               roofline=roofline(alg, ms, traffic),
                plain_csr_gflops=_gf(flops, ms_csr), vendor_csr_gflops=_gf(flops, ms_vendor),
                sparsetir_hyb_gflops=_gf(flops, ms_hyb),
                hybrid_over_csr=round(ms_csr / ms, 3),
                hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),
                hybrid_over_sparsetir_hyb=round(ms_hyb / ms, 3),
+               b_fp32={"us": round(ms_f32 * 1e3, 2), "gflops": _gf(flops, ms_f32),
+                       "hbm_frac": roofline(spmm_alg_bytes_blocks(nblocks, A.nnz, M, K, N, s_b=4), ms_f32)["frac"]},
                mfma=mfma_use(info["num_chunks"], N, ms),
                n128=n128, launch_floor_us=round(ms_floor * 1e3, 2),
                hbm_copy_gbs={"value": round(copy_gbs, 1), "what": "256 MiB device copy (read + write)",
-                             "achieved_over_copy": round(spmm_alg_bytes_blocks(nblocks, A.nnz, M, K, N)
-                                                         / (ms * 1e-3) / 1e9 / copy_gbs, 4)},
+                             "achieved_over_copy": round(alg / (ms * 1e-3) / 1e9 / copy_gbs, 4)},
                plan={"windows": info["num_windows"], "mfma_chunks": info["num_chunks"],
                      "block_nnz": info["block_nnz"], "sparse_nnz": info["sparse_nnz"],
                      "search_tune_and_plan_s": round(t_plan, 3), "tuning": tuning})
     if not args.no_cpu:
-        out["max_rel_err_vs_fp64"] = ops.max_rel_err(C_h[:, :256], ops.spmm_fp64(A, B[:, :256]))
+        B64 = Bd.double().cpu().numpy()
+        out["max_rel_err_vs_fp64"] = ops.max_rel_err(C_h[:, :256], ops.spmm_fp64(A, B64[:, :256]))
         out["cpu_baseline"] = cpu_spmm_baseline(A, B, 512, 10.0, "3072x768")
         out["cpu_torch_csr_all_threads"] = cpu_torch_csr_baseline(A, B, 512, 5.0)
+    if not args.no_batch:
+        out["pruned_batch_n128"] = pruned_batch_summary(args, dev)
     return out
 
 
-def run_pruned_batch(args, dev):
+def pruned_batch_summary(args, dev):
+    """The pruned encoder batch at N = 128 (BF16 B) as a sub-object of the
+    default line: one launch over 120 matrices."""
+    r = run_pruned_batch(args, dev, summary=True)
+    keep = ("ms_per_step", "value", "roofline", "hybrid_over_csr", "plan", "max_rel_err_vs_fp64_3_matrices")
+    return {k: r[k] for k in keep if k in r}
+
+
+def run_pruned_batch(args, dev, summary=False):
     """Many uneven BF16-block matrices in one launch at N = 128: the
     block-pruned projections of a 24-layer BERT-large-shaped encoder (120
     matrices, K = 1024 each), one block-diagonal plan on the B-stationary
@@ -326,7 +348,7 @@ def run_pruned_batch(args, dev):
     t0 = time.perf_counter()
     mats, shapes = datasets.pruned_encoder()
     progress(f"pruned_batch: {len(mats)} matrices, nnz {sum(A.nnz for A in mats)}")
-    plan, _, tuning = api.tune_plan("spmm", mats, "bfloat16", N, log=progress,
+    plan, _, tuning = api.tune_plan("spmm", mats, "bfloat16", N, log=progress, b_dtype="bfloat16",
                                     cache=cache_path(args, "pruned_batch_n128"))
     t_plan = time.perf_counter() - t0
     info = plan.info()
@@ -334,18 +356,19 @@ def run_pruned_batch(args, dev):
     nnz = sum(A.nnz for A in mats)
     g = torch.Generator(device=dev)
     g.manual_seed(0)
-    Bd = torch.randn(Kt, N, device=dev, generator=g)
+    Bf = torch.randn(Kt, N, device=dev, generator=g)
+    Bd = Bf.to(torch.bfloat16)
     C = torch.empty((Mt, N), device=dev, dtype=torch.float32)
     ms = time_launches(lambda: plan.run_spmm(Bd, C), args.steps, args.warmup)
     big = scipy.sparse.block_diag(mats, format="csr").astype(np.float32)
-    ms_csr, ms_vendor = _spmm_comparators(big, Bd, max(args.steps // 4, 10), 3)
-    alg = sum(spmm_alg_bytes_blocks(full_blocks(A), A.nnz, m, k, N) for A, (m, k) in zip(mats, shapes))
+    ms_csr, ms_vendor = _spmm_comparators(big, Bf, max(args.steps // 4, 10), 3)
+    alg = sum(spmm_alg_bytes_blocks(full_blocks(A), A.nnz, m, k, N, s_b=2) for A, (m, k) in zip(mats, shapes))
     flops = 2 * nnz * N
     out = line("SpMM GFLOP/s (2*nnz*N), many BF16-block matrices in one launch at N=128",
                flops / (ms * 1e-3) / 1e9, "GFLOP/s", ms, args, "bf16 MFMA blocks + f32 sparse, f32 accumulate",
                {"workload": f"{len(mats)} block-pruned projections of a 24-layer BERT-large-shaped encoder "
                             f"(Q, K, V, O 1024x1024 and FFN-in 4096x1024 per layer; 10% of 16x16 blocks + 1% "
-                            f"residual; nnz={nnz}) x B 1024x128 each, one block-diagonal launch",
+                            f"residual; nnz={nnz}) x B 1024x128 BF16 each, one block-diagonal launch",
                 "matrices": len(mats), "M_total": Mt, "K_total": Kt, "N": N, "nnz": int(nnz),
                 "parallelism": "single GPU"},
                roofline=roofline(alg, ms, measured_traffic("hyb_spmm_traffic_pruned_batch.json")),
@@ -356,8 +379,8 @@ def run_pruned_batch(args, dev):
                plan={"windows": info["num_windows"], "mfma_chunks": info["num_chunks"],
                      "block_nnz": info["block_nnz"], "sparse_nnz": info["sparse_nnz"],
                      "search_tune_and_plan_s": round(t_plan, 3), "tuning": tuning})
-    if not args.no_cpu:
-        Bh = Bd.cpu().numpy()
+    if not args.no_cpu or summary:
+        Bh = Bd.double().cpu().numpy()
         Ch = C.cpu().numpy()
         errs = []
         r0 = c0 = 0
@@ -366,7 +389,9 @@ def run_pruned_batch(args, dev):
                 errs.append(ops.max_rel_err(Ch[r0:r0 + m], ops.spmm_fp64(A, Bh[c0:c0 + k])))
             r0, c0 = r0 + m, c0 + k
         out["max_rel_err_vs_fp64_3_matrices"] = max(errs)
-        out["cpu_baseline"] = cpu_spmm_baseline(mats[4], Bh[4 * 1024:5 * 1024], N, 10.0, "4096x1024 FFN-in")
+        if not summary and not args.no_cpu:
+            out["cpu_baseline"] = cpu_spmm_baseline(mats[4], Bh[4 * 1024:5 * 1024].astype(np.float32), N, 10.0,
+                                                    "4096x1024 FFN-in")
     return out
 
 
@@ -613,7 +638,9 @@ def main():
     ap.add_argument("--workload", default="pruned_layer", choices=sorted(WORKLOADS) + ["all"])
     ap.add_argument("--gcn-scale", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-n128", action="store_true", help="pruned layer: skip the N=128 secondary line")
+    ap.add_argument("--no-n128", action="store_true", help="pruned layer: skip the N=128 secondary figures")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="pruned layer: skip the 120-matrix encoder batch sub-object")
     ap.add_argument("--plan-cache", default=None,
                     help="directory of tuned packs: used if present (no search), written otherwise")
     args = ap.parse_args()

@@ -506,13 +506,14 @@ TUNE_TOL = 1e-2
 
 
 def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps=20, workers=None, log=None,
-              cache=None):
+              cache=None, b_dtype="float32"):
     """Build the plan of `mats` (one matrix, or a list for a batched plan)
     under each candidate search setting, check each plan's output against a
     float64 product on a sample (relative error <= TUNE_TOL, else the
     candidate is dropped), time the rest and return (fastest plan, its pack,
-    report).  kind: 'spmm' (width = N) or 'sddmm' (width = K_feat, FP32
-    blocks).  report: {"choice": name, "ms": {name: ms per run},
+    report).  kind: 'spmm' (width = N, B of type b_dtype: float32, bfloat16
+    or float16, the type the plan will be run with) or 'sddmm' (width =
+    K_feat, FP32 blocks).  report: {"choice": name, "ms": {name: ms per run},
     "search_s": {name: seconds}, "err": {name: sampled relative error}}.
     cache: an .npz path; if it exists and was saved for this very pattern
     (matrix_fingerprint) its pack is used without search or timing (report
@@ -531,7 +532,7 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
     g = torch.Generator(device=dev)
     g.manual_seed(0)
     if kind == "spmm":
-        Bm = torch.randn(K, width, device=dev, generator=g)
+        Bm = torch.randn(K, width, device=dev, generator=g).to(getattr(torch, str(b_dtype).replace("torch.", "")))
         out = torch.empty(M, width, device=dev)
     else:
         X = torch.randn(M, width, device=dev, generator=g)

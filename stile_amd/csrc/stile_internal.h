@@ -90,6 +90,14 @@ struct SpmmPlanDev {
   const int32_t* srow_p1;
   // parts of huge window w (deterministic mode): [hw_p0[w], hw_p0[w + 1])
   const int32_t* hw_p0;      // [W_huge + 1]
+  // Sparse parts of the dense windows in rounds, for the B-stationary
+  // kernel: window w in [0, W_tc) holds (sp_off[w + 1] - sp_off[w]) / 16
+  // rounds; entry sp_ent[sp_off[w] + 16 r + s] = (column, value bits) of the
+  // r-th sparse nonzero of window slot s (rows in column order, as in
+  // res_cols), or (the matrix's first column, 0) past the slot's end.
+  // nullptr when the plan does not qualify for that kernel.
+  const int32_t* sp_off;     // [W_tc + 1]
+  const int2* sp_ent;
   const int32_t* win_rows;   // [W*16]
   const int32_t* tc_ptr;     // [W+1]
   const int32_t* tc_cols;    // [nchunks*32]
@@ -166,7 +174,8 @@ hipError_t launch_hyb_sddmm(const SddmmPlanDev& p, const float* X,
 // vals[tc_ids[i]] (0 when -1), converted to the plan's block type; sparse
 // entry j takes vals[res_ids[j]].
 hipError_t launch_set_values(const SpmmPlanDev& p, const float* vals, const int32_t* tc_ids, int64_t n_tc,
-                             const int32_t* res_ids, int64_t n_res, hipStream_t s);
+                             const int32_t* res_ids, int64_t n_res, const int32_t* sp_ids, int64_t n_sp,
+                             hipStream_t s);
 
 hipError_t launch_attention(const SpmmPlanDev& p, const void* Q, const void* K, const void* V, int64_t ld,
                             float scale, float* out, int64_t ldo, hipStream_t s);

@@ -826,9 +826,11 @@ __global__ __launch_bounds__(256) void hyb_spmm_sparse_split_kernel(SpmmPlanDev 
 // that lane 4e + i / 4 of the group addresses.  Lane (g, 4e' + q) addresses
 // row tc_cols[c][g][e'] (first read) and tc_cols[c][g][e' + 4] (second read)
 // at position 16t + 4q, so lane j of group g receives B[col(g, e)][n0 + 4j +
-// t] for e = 0..7 -- its MFMA B operand, with no repacking in registers.  The
-// 32 padding bytes per row put the 32 bytes one row contributes to a read at
-// bank offset 40 r + 8 t (mod 64 banks): the 8 rows of a half-wave fall on
+// t] for e = 0..7 -- its MFMA B operand, with no repacking in registers, and
+// the lane's four tiles hold four consecutive columns (each C store
+// instruction writes four rows of 64 consecutive floats).  The 32 padding
+// bytes per row put the 32 bytes one row contributes to a read at bank
+// offset 40 r + 8 t (mod 64 banks): the 8 rows of a half-wave fall on
 // distinct banks when their row ids differ mod 8, which the plan arranges
 // (k-slot order within each chunk).
 // ---------------------------------------------------------------------------
@@ -852,74 +854,82 @@ typedef short i16x4 __attribute__((ext_vector_type(4)));
 typedef short i16x8 __attribute__((ext_vector_type(8)));
 typedef __attribute__((address_space(3))) unsigned char lds_u8;
 typedef __attribute__((address_space(3))) i16x4 lds_i16x4;
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-typedef __attribute__((address_space(3))) u32x2 lds_u32x2;
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 __device__ __forceinline__ i16x4 lds_tr_read(lds_u8* p) {
   return __builtin_amdgcn_ds_read_tr16_b64_v4i16(reinterpret_cast<lds_i16x4*>(p));
 }
 
-// Stage B[koff : koff + ksz, n0 : n0 + 64] into the LDS rows.  Work item
-// (row r, quarter q) covers columns n0 + 16q .. + 15 and writes, for t =
-// 0..3, the 8 bytes at position 16t + 4q: columns 16q + t, + 4, + 8, + 12.
-// Every thread loads all its items (at most 4: ksz <= 1024) before the
-// first write, so the staging costs one memory latency.  vec16: 16-bit B
-// rows aligned for 16-byte loads and N % 8 == 0; vec: as load_b4.
+// Stage B[koff : koff + ksz, n0 : n0 + 64] into the LDS rows.  Consecutive
+// threads take consecutive 16-byte pieces of the block, so every load
+// instruction reads whole rows; each thread loads all its pieces (at most
+// 8 of 16-bit B, 12 of FP32 B per round: one round up to K = 768) before
+// writing them.  Columns at or beyond N are staged as 0.
+//   16-bit B (vec16): piece (row r, columns 8b .. 8b + 7) -> for t = 0..3 the
+//     pair of columns 8b + t, 8b + 4 + t to position 16t + 4(b / 2) + 2(b % 2)
+//     (one v_perm_b32 and one 4-byte write each);
+//   FP32 B: piece (row r, columns 4m .. 4m + 3) -> column 4m + t to position
+//     16t + m (two packed conversions, four 2-byte writes).
 template <typename TB>
 __device__ __forceinline__ void stage_b(lds_u8* lds, const TB* __restrict__ B, int64_t ldb, int koff, int ksz,
                                         int n0, int N, bool vec, bool vec16) {
-  constexpr int U = 4;
-  const int items = ksz * 4;
-  for (int base = threadIdx.x; base < items; base += U * kLdsThreads) {
-    uint32_t w[U][8];  // w[u][2t + h]: columns (8h + t, 8h + 4 + t) of the item as a BF16 pair
-    if constexpr (std::is_same<TB, __bf16>::value) {
-      if (vec16) {
-        uint4 d[U][2];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int it = min(base + u * kLdsThreads, items - 1);
-          const int c0 = n0 + 16 * (it & 3);
-          const uint4* src = reinterpret_cast<const uint4*>(B + (int64_t)(koff + (it >> 2)) * ldb + c0);
-          d[u][0] = c0 < N ? src[0] : make_uint4(0, 0, 0, 0);
-          d[u][1] = c0 + 8 < N ? src[1] : make_uint4(0, 0, 0, 0);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const uint32_t dd[8] = {d[u][0].x, d[u][0].y, d[u][0].z, d[u][0].w,
-                                  d[u][1].x, d[u][1].y, d[u][1].z, d[u][1].w};
-#pragma unroll
-          for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) w[u][2 * t + h] = half_sel(dd[4 * h + t / 2], dd[4 * h + 2 + t / 2], t & 1);
-        }
-        goto write;
-      }
-    }
-    {
-      float4 f[U][4];
+  if constexpr (!std::is_same<TB, float>::value) {
+    if (vec16) {
+      constexpr int U = kLdsMaxK * 8 / kLdsThreads;
+      const int items = ksz * 8;
+      uint4 d[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int it = min(base + u * kLdsThreads, items - 1);
-        const int c0 = n0 + 16 * (it & 3);
-#pragma unroll
-        for (int m = 0; m < 4; ++m) f[u][m] = load_b4(B, ldb, koff + (it >> 2), c0 + 4 * m, N, vec);
+        const int it = min((int)threadIdx.x + u * kLdsThreads, items - 1);
+        const int c0 = n0 + 8 * (it & 7);
+        d[u] = c0 < N ? *reinterpret_cast<const uint4*>(B + (int64_t)(koff + (it >> 3)) * ldb + c0)
+                      : make_uint4(0, 0, 0, 0);
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u)
+      for (int u = 0; u < U; ++u) {
+        const int it = threadIdx.x + u * kLdsThreads;
+        if (it < items) {
+          uint32_t dd[4] = {d[u].x, d[u].y, d[u].z, d[u].w};
+          if constexpr (std::is_same<TB, _Float16>::value) {
+            const float4 lo = cvt4<TB>(make_uint2(dd[0], dd[1])), hi = cvt4<TB>(make_uint2(dd[2], dd[3]));
+            dd[0] = bf16_pair(lo.x, lo.y);
+            dd[1] = bf16_pair(lo.z, lo.w);
+            dd[2] = bf16_pair(hi.x, hi.y);
+            dd[3] = bf16_pair(hi.z, hi.w);
+          }
+          const int b = it & 7;
+          lds_u8* row = lds + (it >> 3) * kLdsRowBytes + 8 * (b >> 1) + 4 * (b & 1);
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-          for (int h = 0; h < 2; ++h) w[u][2 * t + h] = bf16_pair(comp(f[u][2 * h], t), comp(f[u][2 * h + 1], t));
+          for (int t = 0; t < 4; ++t)
+            *reinterpret_cast<lds_u32*>(row + 32 * t) = half_sel(dd[t >> 1], dd[2 + (t >> 1)], t & 1);
+        }
+      }
+      return;
     }
-  write:
+  }
+  // 12 loads of 16 bytes in flight per thread: one round up to K = 768
+  // (48 registers; the persistent variants have no room for 64); 8 for the
+  // unaligned 16-bit fallback, which shares the kernel with the 16-byte path
+  constexpr int U = std::is_same<TB, float>::value ? 12 : 8;
+  const int items = ksz * 16;
+  for (int base = threadIdx.x; base < items; base += U * kLdsThreads) {
+    float4 f[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int it = min(base + u * kLdsThreads, items - 1);
+      f[u] = load_b4(B, ldb, koff + (it >> 4), n0 + 4 * (it & 15), N, vec);
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int it = base + u * kLdsThreads;
       if (it < items) {
-        lds_u8* row = lds + (it >> 2) * kLdsRowBytes + 8 * (it & 3);
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-          *reinterpret_cast<lds_u32x2*>(row + 32 * t) = u32x2{w[u][2 * t], w[u][2 * t + 1]};
+        lds_u8* pos = lds + (it >> 4) * kLdsRowBytes + 2 * (it & 15);
+        const uint32_t p01 = bf16_pair(f[u].x, f[u].y), p23 = bf16_pair(f[u].z, f[u].w);
+        *reinterpret_cast<lds_u16*>(pos) = (uint16_t)p01;
+        *reinterpret_cast<lds_u16*>(pos + 32) = (uint16_t)(p01 >> 16);
+        *reinterpret_cast<lds_u16*>(pos + 64) = (uint16_t)p23;
+        *reinterpret_cast<lds_u16*>(pos + 96) = (uint16_t)(p23 >> 16);
       }
     }
   }
@@ -956,38 +966,54 @@ __device__ __forceinline__ void tr_chunk_mfma(lds_u8* lds, int cst, const TrChun
   }
 }
 
-// Rows of the lane group's 4 window slots and, in plans with mixed windows
-// (kMixed), the bounds of their sparse parts.
-template <bool kMixed>
-struct WinRegs {
-  SlotRows sr;
-  int rp[kMixed ? 5 : 1];
-};
+// Rows of the lane group's 4 window slots.
+__device__ __forceinline__ SlotRows load_win(const SpmmPlanDev& p, int slot0) { return load_rows(p, slot0); }
 
-template <bool kMixed>
-__device__ __forceinline__ WinRegs<kMixed> load_win(const SpmmPlanDev& p, int slot0) {
-  WinRegs<kMixed> r;
-  r.sr = load_rows(p, slot0);
-  if constexpr (kMixed) {
+// One round of a window's sparse part (kMixed plans): e holds the round's
+// entry of slot 4g + (lane & 3) (column, value bits); lane (g, j) adds
+// v * B[col][n0 + 4j + t] of the group's four slots to acc[t][i].  BF16 B is
+// read from the staged LDS block (position 16t + j of row col - koff), other
+// B types from memory (FP32 products on B's exact values either way).
+template <typename TB>
+__device__ __forceinline__ void sparse_round(const TB* __restrict__ B, int64_t ldb, int N, int col, bool vec,
+                                             lds_u8* lds, int koff, int2 e, f32x4 (&acc)[4]) {
+  const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < 5; ++i) r.rp[i] = p.res_ptr[slot0 + i];
+  for (int i = 0; i < 4; ++i) {
+    const int src = (lane & 48) + i;
+    const int k = __shfl(e.x, src);
+    const float v = __int_as_float(__shfl(e.y, src));
+    float4 b;
+    if constexpr (std::is_same<TB, __bf16>::value) {
+      lds_u8* rp = lds + (k - koff) * kLdsRowBytes + 2 * (lane & 15);
+      b.x = bf16_lo(*reinterpret_cast<lds_u16*>(rp));
+      b.y = bf16_lo(*reinterpret_cast<lds_u16*>(rp + 32));
+      b.z = bf16_lo(*reinterpret_cast<lds_u16*>(rp + 64));
+      b.w = bf16_lo(*reinterpret_cast<lds_u16*>(rp + 96));
+    } else {
+      b = load_b4(B, ldb, k, col, N, vec);
+    }
+    acc[0][i] += v * b.x;
+    acc[1][i] += v * b.y;
+    acc[2][i] += v * b.z;
+    acc[3][i] += v * b.w;
   }
-  return r;
 }
 
 // One work item (unit item / nC, column chunk item % nC) of the B-stationary
-// kernel.  tc_ptr is p.tc_ptr, passed again as a restrict parameter so that
-// the compiler may read it with scalar loads inside the loop that stores C.
-// Returns early per wave (a wave with no windows) or per workgroup (an empty
-// unit); the caller's barrier before the next item's staging keeps the LDS
-// block until every wave is done with it.  kMixed: windows may have sparse
-// parts, which are added (whole rows, four rows of a lane group at a time)
-// before the window's rows are stored.
+// kernel.  tc_ptr and sp_off are p.tc_ptr and p.sp_off, passed again as
+// restrict parameters so that the compiler may read them with scalar loads
+// inside the loop that stores C.  Returns early per wave (a wave with no
+// windows) or per workgroup (an empty unit); the caller's barrier before the
+// next item's staging keeps the LDS block until every wave is done with it.
+// kMixed: windows may have sparse parts, added (in rounds, p.sp_ent) to the
+// dense sums before the window's rows are stored; round 0 of a window is
+// requested one window ahead, the others when the window ends.
 template <typename TB, bool kMixed>
 __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __restrict__ tc_ptr,
-                                         const int4* __restrict__ units, const TB* __restrict__ B,
-                                         int64_t ldb, int N, float* __restrict__ C, int64_t ldc, int nC,
-                                         bool vec, bool vec16, int item, lds_u8* lds) {
+                                         const int32_t* __restrict__ sp_off, const int4* __restrict__ units,
+                                         const TB* __restrict__ B, int64_t ldb, int N, float* __restrict__ C,
+                                         int64_t ldc, int nC, bool vec, bool vec16, int item, lds_u8* lds) {
   constexpr int kWaves = kLdsThreads / 64;
   const int chunk = item % nC;
   const int4 unit = units[item / nC];
@@ -999,9 +1025,10 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   // every step wait for all outstanding vector loads and stores.
   const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = lane >> 4, j = lane & 15;
-  const int col = n0 + 4 * j;
+  const int col = n0 + 4 * j;  // the lane's columns: col .. col + 3
   const int cidx = g * 8 + ((lane >> 2) & 3);
   const int cst = 8 * (lane & 3) - koff * kLdsRowBytes;
+  const int eidx = 4 * g + (lane & 3);  // slot of the round entry this lane requests
   // This wave's contiguous run of the unit's windows, balanced by chunk count.
   const int wg_beg = unit.x;
   const int wg_end = unit.y;
@@ -1031,13 +1058,20 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   const int last = c_end - 1;
   int w = w_beg;
   int w_next = 0;
-  WinRegs<kMixed> wr{};
+  SlotRows wr{};
   TrChunk r0{}, r1{};
+  int so0 = 0, so1 = 0;  // sp_off of the current window and the next (kMixed)
+  int2 e0 = make_int2(0, 0);
   if (has) {
     w_next = tc_ptr[w + 1];
-    wr = load_win<kMixed>(p, w * 16 + g * 4);
+    wr = load_win(p, w * 16 + g * 4);
     r0 = load_tr_chunk(p, c_beg, cidx, lane);
     r1 = load_tr_chunk(p, min(c_beg + 1, last), cidx, lane);
+    if constexpr (kMixed) {
+      so0 = sp_off[w];
+      so1 = sp_off[w + 1];
+      if (so1 > so0) e0 = p.sp_ent[so0 + eidx];
+    }
   }
   stage_b(lds, B, ldb, koff, ksz, n0, N, vec, vec16);
   __syncthreads();
@@ -1045,35 +1079,40 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   f32x4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  auto finish = [&](const WinRegs<kMixed>& cur) {
+  auto finish = [&](const SlotRows& cur) {
     if constexpr (kMixed) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int a = cur.rp[i], b = cur.rp[i + 1];
-        if (a < b) {
-          const float4 s = sparse_row_sum<4>(p, B, ldb, N, col, vec, a, b, b - a > kHubNnz);
-          acc[0][i] += s.x;
-          acc[1][i] += s.y;
-          acc[2][i] += s.z;
-          acc[3][i] += s.w;
-        }
+      const int rounds = (so1 - so0) >> 4;  // wave-uniform
+      if (rounds > 0) {
+        sparse_round(B, ldb, N, col, vec, lds, koff, e0, acc);
+        for (int r = 1; r < rounds; ++r) sparse_round(B, ldb, N, col, vec, lds, koff, p.sp_ent[so0 + 16 * r + eidx], acc);
       }
     }
-    store_window<true>(C, ldc, N, col, vec, cur.sr, acc);
+    store_window<true>(C, ldc, N, col, vec, cur, acc);
   };
   TrChunk r2;
   int c = c_beg;
   auto step = [&](const TrChunk& use, TrChunk& fill) {
     fill = load_tr_chunk(p, min(c + 2, last), cidx, lane);
     if (c >= w_next) {  // window w is complete: store it
-      const WinRegs<kMixed> wn = load_win<kMixed>(p, (w + 1) * 16 + g * 4);
+      const SlotRows wn = load_win(p, (w + 1) * 16 + g * 4);
       const int w_next2 = tc_ptr[w + 2];
+      int so2 = 0;
+      int2 e0n = make_int2(0, 0);
+      if constexpr (kMixed) {
+        so2 = sp_off[w + 2];
+        if (so2 > so1) e0n = p.sp_ent[so1 + eidx];
+      }
       finish(wr);
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
       wr = wn;
       ++w;
       w_next = w_next2;
+      if constexpr (kMixed) {
+        so0 = so1;
+        so1 = so2;
+        e0 = e0n;
+      }
     }
     tr_chunk_mfma(lds, cst, use, acc);
     ++c;
@@ -1106,6 +1145,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
 template <bool kLoop, typename TB, bool kMixed>
 __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p,
                                                                         const int32_t* __restrict__ tc_ptr,
+                                                                        const int32_t* __restrict__ sp_off,
                                                                         const int4* __restrict__ units,
                                                                         const TB* __restrict__ B,
                                                                         int64_t ldb, int N, float* __restrict__ C,
@@ -1117,7 +1157,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   const int tb = (int)gridDim.x - 1 - (int)blockIdx.x;
   if (!kLoop) {
     if ((int)blockIdx.x < n_items)
-      lds_item<TB, kMixed>(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, vec16, blockIdx.x, lds);
+      lds_item<TB, kMixed>(p, tc_ptr, sp_off, units, B, ldb, N, C, ldc, nC, vec, vec16, blockIdx.x, lds);
     if (tb < tail_blocks) {
       const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
       if (task < tail_tasks) sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, task);
@@ -1140,7 +1180,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   }
   for (int item = beg; item < end; item += step) {
     __syncthreads();  // every wave is done reading the previous item's block
-    lds_item<TB, kMixed>(p, tc_ptr, units, B, ldb, N, C, ldc, nC, vec, vec16, item, lds);
+    lds_item<TB, kMixed>(p, tc_ptr, sp_off, units, B, ldb, N, C, ldc, nC, vec, vec16, item, lds);
   }
   if (tb < tail_blocks) {
     const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
@@ -1357,10 +1397,10 @@ static hipError_t launch_hyb_spmm_t(const SpmmPlanDev& p, const TB* B, int64_t l
     if (items <= cus) grid = std::max(grid, tail_blocks);
     const size_t lds_bytes = (size_t)p.lds_max_k * kLdsRowBytes;
     const int4* units = p.lds_units + p.lds_off[t];
-    const bool mixed = p.n_mix > 0;
+    const bool mixed = p.n_mix > 0 && p.sp_off != nullptr;
 #define STILE_LDS_LAUNCH(LOOP, MIX)                                                                          \
   hipLaunchKernelGGL((hyb_spmm_lds_bf16_kernel<LOOP, TB, MIX>), dim3(grid), dim3(kLdsThreads), lds_bytes, s, p, \
-                     p.tc_ptr, units, B, ldb, N, C, ldc, nC, (int)items, vec, vec16, xr, LOOP && xcd,          \
+                     p.tc_ptr, p.sp_off, units, B, ldb, N, C, ldc, nC, (int)items, vec, vec16, xr, LOOP && xcd, \
                      tail_blocks, tail_tasks)
     if (items <= cus) {
       if (mixed) STILE_LDS_LAUNCH(false, true); else STILE_LDS_LAUNCH(false, false);
@@ -1914,8 +1954,9 @@ __global__ __launch_bounds__(256) void set_values_kernel(const float* __restrict
                                                          const int32_t* __restrict__ tc_ids, int64_t n_tc,
                                                          void* tc_vals, int mode,
                                                          const int32_t* __restrict__ res_ids, int64_t n_res,
-                                                         float* res_vals) {
-  const int64_t n = n_tc + n_res;
+                                                         float* res_vals, const int32_t* __restrict__ sp_ids,
+                                                         int64_t n_sp, int2* sp_ent) {
+  const int64_t n = n_tc + n_res + n_sp;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     if (i < n_tc) {
       const int id = tc_ids[i];
@@ -1924,21 +1965,28 @@ __global__ __launch_bounds__(256) void set_values_kernel(const float* __restrict
         reinterpret_cast<__bf16*>(tc_vals)[i] = (__bf16)v;
       else
         reinterpret_cast<float*>(tc_vals)[i] = v;
-    } else {
+    } else if (i < n_tc + n_res) {
       const int64_t j = i - n_tc;
       res_vals[j] = vals[res_ids[j]];
+    } else {
+      // the value half of a round entry; padding entries (id -1) stay 0
+      const int64_t j = i - n_tc - n_res;
+      const int id = sp_ids[j];
+      reinterpret_cast<float*>(sp_ent)[2 * j + 1] = id >= 0 ? vals[id] : 0.f;
     }
   }
 }
 
 hipError_t launch_set_values(const SpmmPlanDev& p, const float* vals, const int32_t* tc_ids, int64_t n_tc,
-                             const int32_t* res_ids, int64_t n_res, hipStream_t s) {
-  const int64_t n = n_tc + n_res;
+                             const int32_t* res_ids, int64_t n_res, const int32_t* sp_ids, int64_t n_sp,
+                             hipStream_t s) {
+  const int64_t n = n_tc + n_res + n_sp;
   if (n == 0) return hipSuccess;
   int64_t blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(set_values_kernel, dim3((unsigned)blocks), dim3(256), 0, s, vals, tc_ids, n_tc,
-                     const_cast<void*>(p.tc_vals), p.blk_dtype, res_ids, n_res, const_cast<float*>(p.res_vals));
+                     const_cast<void*>(p.tc_vals), p.blk_dtype, res_ids, n_res, const_cast<float*>(p.res_vals),
+                     sp_ids, n_sp, const_cast<int2*>(p.sp_ent));
   return hipGetLastError();
 }
 

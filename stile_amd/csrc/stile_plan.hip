@@ -39,6 +39,8 @@ struct StilePlan {
   int64_t n_tc_slots = 0, n_res = 0;
   const int32_t* tc_ids = nullptr;
   const int32_t* res_ids = nullptr;
+  int64_t n_sp = 0;
+  const int32_t* sp_ids = nullptr;
   // per-XCD item ranges of the B-stationary kernel (spmm.lds_xcd_host)
   std::vector<int32_t> lds_xcd;
 };
@@ -484,37 +486,60 @@ void build_lds_units(const SpmmBuild& bd, const std::vector<int32_t>& tc_ptr, in
 // k-slot order within each chunk of 32 dense columns of a window, for the
 // B-stationary kernel: k-slot ks = 4e + g is read by lane group g, and one
 // transposed LDS read of a half-wave gathers the B rows of the 8 k-slots
-// {16h + 4e' + g : e' = 0..3, g in {2P, 2P + 1}} (h, P = 0, 1).  Those rows
-// fall on distinct LDS banks when they differ mod 8 (rows of kLdsRowBytes
-// bytes).  The columns are dealt to the four k-slot sets by residue class
-// (col - koff) mod 8, so every set holds distinct residues whenever no class
-// has more than 4 of the chunk's columns (two 16-column dense blocks: 2
-// each).  Only the order changes; the last chunk keeps its real columns in
-// its first n k-slots.
+// {16h + 4e' + g : e' = 0..3, g in {2P, 2P + 1}} (h, P = 0, 1), each lane
+// group's 4 rows in one 16-lane access.  Rows of kLdsRowBytes bytes fall on
+// distinct LDS banks when a lane group's 4 rows differ mod 4 and the
+// half-wave's 8 rows differ mod 8.  The columns are dealt to the four k-slot
+// sets by residue class (col - koff) mod 8, so every set holds distinct
+// residues whenever no class has more than 4 of the chunk's columns (two
+// 16-column dense blocks: 2 each); within a set the lower residues go to
+// lane group 2P and the higher to 2P + 1 ({0..3} and {4..7} for a full set).
+// Only the order changes; the last chunk keeps its real columns in its
+// first n k-slots.  (A transposed read whose 8 half-wave rows share a
+// residue costs 7 extra LDS cycles; with distinct residues none.)
 void order_chunk_slots(std::vector<DenseCol>& cols, int64_t koff) {
+  // Which columns share a chunk is free (a window's chunks together cover
+  // its columns): deal the columns to the chunks round-robin over the
+  // residue classes, so each chunk of 32 holds at most 4 of a class while
+  // every class still has columns left.
+  {
+    std::vector<DenseCol> by[8];
+    for (const DenseCol& dc : cols) by[(dc.col - koff) & 7].push_back(dc);
+    size_t at[8] = {0, 0, 0, 0, 0, 0, 0, 0}, o = 0;
+    while (o < cols.size())
+      for (int r = 0; r < 8; ++r)
+        if (at[r] < by[r].size()) cols[o++] = by[r][at[r]++];
+  }
   for (size_t base = 0; base < cols.size(); base += 32) {
     const int n = (int)std::min<size_t>(32, cols.size() - base);
-    // the real k-slots of each set
+    // the real k-slots of each set, lane group first (g, then e')
     std::vector<int> slots[4];
     for (int ks = 0; ks < n; ++ks) slots[(ks >> 4) * 2 + ((ks & 3) >> 1)].push_back(ks);
+    for (auto& sl : slots)
+      std::stable_sort(sl.begin(), sl.end(), [](int a, int b) { return (a & 3) < (b & 3); });
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
     auto res = [&](int i) { return (int)((cols[base + i].col - koff) & 7); };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return res(a) < res(b); });
     unsigned used[4] = {0, 0, 0, 0};  // residues present per set
-    size_t fill[4] = {0, 0, 0, 0};
-    std::vector<DenseCol> out(n);
+    std::vector<int> members[4];      // chunk column indices dealt to each set
     for (int i = 0; i < n; ++i) {
       const int r = res(order[i]);
       int pick = -1;
       for (int d = 0; d < 4 && pick < 0; ++d) {  // preferred set first, then the others
         const int set = (i + d) & 3;
-        if (fill[set] < slots[set].size() && !(used[set] >> r & 1u)) pick = set;
+        if (members[set].size() < slots[set].size() && !(used[set] >> r & 1u)) pick = set;
       }
       for (int set = 0; set < 4 && pick < 0; ++set)
-        if (fill[set] < slots[set].size()) pick = set;
+        if (members[set].size() < slots[set].size()) pick = set;
       used[pick] |= 1u << r;
-      out[slots[pick][fill[pick]++]] = cols[base + order[i]];
+      members[pick].push_back(order[i]);
+    }
+    std::vector<DenseCol> out(n);
+    for (int set = 0; set < 4; ++set) {
+      // ascending residue: the lower half to the set's first lane group
+      std::stable_sort(members[set].begin(), members[set].end(), [&](int a, int b) { return res(a) < res(b); });
+      for (size_t m = 0; m < members[set].size(); ++m) out[slots[set][m]] = cols[base + members[set][m]];
     }
     std::copy(out.begin(), out.end(), cols.begin() + base);
   }
@@ -671,6 +696,33 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   if (lds_ok && W_tc > 0)
     build_lds_units(bd, tc_ptr, W_tc, stile::device_cus(), lds_units, lds_off, lds_cnt, lds_max_k, lds_pick,
                     lds_xcd);
+  // Sparse parts of the dense windows in rounds (B-stationary plans).
+  std::vector<int32_t> sp_off, sp_ids;
+  std::vector<int32_t> sp_ent;  // (column, value bits) pairs
+  if (lds_ok && W_tc > 0) {
+    sp_off.assign((size_t)W_tc + 1, 0);
+    int64_t total = 0;
+    for (int32_t w = 0; w < W_tc; ++w) {
+      sp_off[w] = (int32_t)total;
+      int64_t R = 0;
+      for (int sl = 0; sl < 16; ++sl) R = std::max<int64_t>(R, cnt[(size_t)w * 16 + sl + 1] - cnt[(size_t)w * 16 + sl]);
+      const int32_t pad_col = (int32_t)bd.inst_cols[window_matrix(bd, w)];
+      for (int64_t r = 0; r < R; ++r)
+        for (int sl = 0; sl < 16; ++sl) {
+          const int64_t q = cnt[(size_t)w * 16 + sl] + r;
+          const bool real = q < cnt[(size_t)w * 16 + sl + 1];
+          const float v = real ? res_vals[q] : 0.f;
+          int32_t vb;
+          std::memcpy(&vb, &v, 4);
+          sp_ent.push_back(real ? res_cols[q] : pad_col);
+          sp_ent.push_back(vb);
+          if (bd.has_ids) sp_ids.push_back(real ? res_ids[q] : -1);
+        }
+      total += R * 16;
+      if (total > INT32_MAX) return fail(STILE_E_UNSUPPORTED, "too many sparse entries in dense windows");
+    }
+    sp_off[W_tc] = (int32_t)total;
+  }
   // Parts of the huge windows: consecutive ranges of at most kHugeChunks chunks.
   std::vector<int32_t> part_win, part_c0, part_c1, hw_p0;
   for (int32_t w = 0; w < W_huge; ++w) {
@@ -763,6 +815,9 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const size_t o_sp0 = lay.add(srow_p0.size() * 4);
   const size_t o_sp1 = lay.add(srow_p1.size() * 4);
   const size_t o_hwp0 = lay.add(hw_p0.size() * 4);
+  const size_t o_spoff = lay.add(sp_off.size() * 4);
+  const size_t o_spent = lay.add(sp_ent.size() * 4);
+  const size_t o_spids = lay.add(sp_ids.size() * 4);
   HostBlob blob;
   blob.data.assign(lay.bytes, 0);
   blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
@@ -788,6 +843,9 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   blob.put(o_sp0, srow_p0.data(), srow_p0.size() * 4);
   blob.put(o_sp1, srow_p1.data(), srow_p1.size() * 4);
   blob.put(o_hwp0, hw_p0.data(), hw_p0.size() * 4);
+  blob.put(o_spoff, sp_off.data(), sp_off.size() * 4);
+  blob.put(o_spent, sp_ent.data(), sp_ent.size() * 4);
+  blob.put(o_spids, sp_ids.data(), sp_ids.size() * 4);
 
   StilePlan* plan = new (std::nothrow) StilePlan();
   if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
@@ -831,6 +889,10 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.srow_p0 = dptr<int32_t>(plan, o_sp0);
   d.srow_p1 = dptr<int32_t>(plan, o_sp1);
   d.hw_p0 = dptr<int32_t>(plan, o_hwp0);
+  d.sp_off = sp_off.empty() ? nullptr : dptr<int32_t>(plan, o_spoff);
+  d.sp_ent = sp_off.empty() ? nullptr : dptr<int2>(plan, o_spent);
+  plan->n_sp = (int64_t)sp_ids.size();
+  plan->sp_ids = sp_ids.empty() ? nullptr : dptr<int32_t>(plan, o_spids);
   d.piece_row = dptr<int32_t>(plan, o_prow);
   d.piece_beg = dptr<int32_t>(plan, o_pbeg);
   d.piece_end = dptr<int32_t>(plan, o_pend);
@@ -1215,7 +1277,7 @@ int stile_plan_set_values(StilePlan* plan, const float* vals, int64_t nnz, void*
   if (nnz <= plan->max_id) return fail(STILE_E_INVALID, "values array shorter than the plan's largest value id");
   if (!vals && (plan->n_tc_slots + plan->n_res) > 0) return fail(STILE_E_INVALID, "null values");
   hipError_t e = stile::launch_set_values(plan->spmm, vals, plan->tc_ids, plan->n_tc_slots, plan->res_ids,
-                                          plan->n_res, (hipStream_t)stream);
+                                          plan->n_res, plan->sp_ids, plan->n_sp, (hipStream_t)stream);
   return e == hipSuccess ? STILE_OK : hip_fail(e, "set_values launch");
 }
 
