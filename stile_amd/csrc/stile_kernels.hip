@@ -1008,7 +1008,7 @@ __device__ __forceinline__ void sparse_round(const TB* __restrict__ B, int64_t l
 // next item's staging keeps the LDS block until every wave is done with it.
 // kMixed: windows may have sparse parts, added (in rounds, p.sp_ent) to the
 // dense sums before the window's rows are stored; round 0 of a window is
-// requested one window ahead, the others when the window ends.
+// requested one window ahead, the others four at a time when it ends.
 template <typename TB, bool kMixed>
 __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __restrict__ tc_ptr,
                                          const int32_t* __restrict__ sp_off, const int4* __restrict__ units,
@@ -1084,7 +1084,17 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
       const int rounds = (so1 - so0) >> 4;  // wave-uniform
       if (rounds > 0) {
         sparse_round(B, ldb, N, col, vec, lds, koff, e0, acc);
-        for (int r = 1; r < rounds; ++r) sparse_round(B, ldb, N, col, vec, lds, koff, p.sp_ent[so0 + 16 * r + eidx], acc);
+        // the other rounds' entries requested four at a time: one memory
+        // latency per four rounds
+        for (int r = 1; r < rounds; r += 4) {
+          int2 e[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            e[u] = r + u < rounds ? p.sp_ent[so0 + 16 * (r + u) + eidx] : make_int2(koff, 0);
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (r + u < rounds) sparse_round(B, ldb, N, col, vec, lds, koff, e[u], acc);
+        }
       }
     }
     store_window<true>(C, ldc, N, col, vec, cur, acc);
