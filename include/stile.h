@@ -150,8 +150,11 @@ typedef struct {
   int64_t block_nnz;      /* nonzeros carried by MFMA blocks */
   int64_t sparse_nnz;     /* nonzeros carried by the FP32 1D/ELL path */
   int64_t device_bytes;   /* bytes of plan buffers resident on the device */
-  int32_t hub_rows;       /* rows with > 4096 sparse nonzeros (compensated sums) */
-  int32_t num_split;      /* SpMM: pieces of hub rows reduced by the fix-up kernel */
+  int32_t hub_rows;       /* rows with > 4096 sparse nonzeros: each 256-term
+                             piece of such a row is summed with Neumaier
+                             compensation, the pieces then plainly added */
+  int32_t num_split;      /* SpMM: pieces of rows with > 256 sparse nonzeros,
+                             added to C by the fix-up kernel */
 } StilePlanInfo;
 
 /* ---------------------------------------------------------------------------
@@ -206,8 +209,12 @@ int stile_csr_row_softmax_f32(int32_t M, const int32_t* indptr, const float* in,
                               void* stream);
 
 /* Build a hybrid SpMM plan from the packed groups (host pointers; copied).
- * The plan regroups every group's products by 16-row output window so one
- * kernel writes each row of C exactly once (no atomics, no zero-fill). */
+ * The plan regroups every group's products by 16-row output window, so C
+ * needs no zero-fill and most rows are stored once with a fixed summation
+ * order.  Two kinds of rows are added with atomicAdd, in completion order,
+ * unless the run passes STILE_RUN_DETERMINISTIC: the pieces of rows with more
+ * than 256 sparse nonzeros, and the parts of windows with more than 256
+ * dense chunks (such windows are zeroed first). */
 int stile_plan_create_spmm(const StileSpmmPack* pack, int32_t blk_dtype,
                            StilePlan** out);
 

@@ -8,12 +8,14 @@
 namespace stile {
 
 // Hub rows: a row whose sparse (non-MFMA) part has more than this many
-// nonzeros is summed with compensated (Neumaier) FP32 arithmetic.
+// nonzeros is summed with compensated (Neumaier) FP32 arithmetic, per
+// kSplitNnz-term piece (the pieces are then added plainly).
 constexpr int kHubNnz = 4096;
 
 // Rows whose sparse part is longer than this are cut: the window pass sums
 // the first kSplitNnz nonzeros, the rest go to pieces of kSplitNnz that a
-// fix-up kernel adds with atomicAdd after the window pass.
+// fix-up kernel adds with atomicAdd after the window pass (in deterministic
+// mode: stores to a workspace, summed in piece order by a reduce kernel).
 constexpr int kSplitNnz = 256;
 // Windows with more than kBigChunks dense chunks are "big": the streaming
 // kernel gives each (big window, 64 columns) a workgroup of kBigWaves waves,
