@@ -253,6 +253,21 @@ int stile_spmm_workspace_bytes(const StilePlan* plan, int32_t N, int32_t flags, 
 int stile_spmm_run_ex(const StilePlan* plan, int32_t b_dtype, const void* B, int64_t ldb, int32_t N, float* C,
                       int64_t ldc, int32_t flags, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* Size of the static device scratch of stile_spmm_run_noalloc (per GPU). */
+#define STILE_NOALLOC_SCRATCH_BYTES (16 << 20)
+
+/* stile_spmm_run_ex that allocates nothing and takes no workspace: with
+ * STILE_RUN_DETERMINISTIC the workspace is a static device array of
+ * STILE_NOALLOC_SCRATCH_BYTES in the library (one per GPU, held until the
+ * process ends), so a call whose stile_spmm_workspace_bytes exceeds that is
+ * rejected with STILE_E_INVALID before any launch.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (several streams or threads) race. */
+int stile_spmm_run_noalloc(const StilePlan* plan, int32_t b_dtype, const void* B, int64_t ldb, int32_t N, float* C,
+                           int64_t ldc, int32_t flags, void* stream);
+
 /* out[p] = <X[row(p)], Y[col(p)]> for every nonzero p of the plan's pattern;
  * X: M x Kf, Y: N x Kf FP32 with row strides ldx, ldy; out has pack->nnz
  * entries in the pattern's CSR order (the reference's remap ids). */

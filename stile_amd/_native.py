@@ -13,6 +13,7 @@ from . import build as _build
 
 STILE_BLK_F32, STILE_BLK_BF16, STILE_BLK_F16 = 0, 1, 2
 STILE_RUN_DETERMINISTIC = 1
+STILE_NOALLOC_SCRATCH_BYTES = 16 << 20
 _BLK_NAMES = {"float32": STILE_BLK_F32, "fp32": STILE_BLK_F32, "f32": STILE_BLK_F32,
               "bfloat16": STILE_BLK_BF16, "bf16": STILE_BLK_BF16,
               "float16": STILE_BLK_F16, "fp16": STILE_BLK_F16, "f16": STILE_BLK_F16}
@@ -98,6 +99,9 @@ _SIGNATURES = {
                                          ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     "stile_spmm_workspace_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                                   ctypes.POINTER(ctypes.c_int64)]),
+    "stile_spmm_run_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                                              ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                              ctypes.c_void_p]),
     "stile_sddmm_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                        ctypes.c_void_p, ctypes.c_void_p]),
@@ -485,6 +489,27 @@ class Plan:
         _check(load().stile_spmm_run_ex(self._h, codes[B.dtype], B.data_ptr(), B.stride(0), N, C.data_ptr(),
                                         C.stride(0), flags, ws_ptr, ws_bytes, _stream(B)))
         return C
+
+    def run_spmm_noalloc(self, B, out, deterministic=False):
+        """C = A.B into `out` through stile_spmm_run_noalloc: no allocation,
+        the deterministic workspace is the library's static device scratch
+        (STILE_NOALLOC_SCRATCH_BYTES per GPU; larger needs are rejected)."""
+        import torch
+        if self.kind != "spmm":
+            raise StileError("not an SpMM plan")
+        _dev_check(B, out)
+        codes = {torch.float32: STILE_BLK_F32, torch.bfloat16: STILE_BLK_BF16, torch.float16: STILE_BLK_F16}
+        if B.dtype not in codes:
+            raise StileError("B must be float32, bfloat16 or float16")
+        if B.shape[0] != self.K or B.stride(1) != 1:
+            raise StileError(f"B must be {self.K} x N with contiguous rows")
+        N = B.shape[1]
+        if out.dtype != torch.float32 or tuple(out.shape) != (self.M, N) or out.stride(1) != 1:
+            raise StileError(f"out must be a float32 {self.M} x {N} tensor with contiguous rows")
+        _check(load().stile_spmm_run_noalloc(self._h, codes[B.dtype], B.data_ptr(), B.stride(0), N, out.data_ptr(),
+                                             out.stride(0), STILE_RUN_DETERMINISTIC if deterministic else 0,
+                                             _stream(B)))
+        return out
 
     def run_sddmm(self, X, Y, out=None):
         import torch

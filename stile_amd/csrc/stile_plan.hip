@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <unordered_map>
@@ -1111,6 +1112,42 @@ int create_sddmm(const StileSddmmPack* packs, int32_t count, int32_t blk_dtype, 
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// stile_spmm_run_noalloc: the deterministic-mode workspace in this code
+// object's own device memory, STILE_NOALLOC_SCRATCH_BYTES per GPU, held
+// until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: the device address comes from
+// hipGetSymbolAddress, once per device with that device current, and only
+// that address is handed to kernels.  The workspace needs no clearing: the
+// pieces and parts kernels store every element before the reduce kernels
+// read it.  Calls that overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) float scratch[STILE_NOALLOC_SCRATCH_BYTES / 4];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxScratchDevices] = {};
+
+int device_scratch(float** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxScratchDevices) return fail(STILE_E_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress of the static scratch");
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return STILE_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int32_t stile_abi_version(void) { return 3; }
@@ -1248,6 +1285,25 @@ int stile_spmm_run_ex(const StilePlan* plan, int32_t b_dtype, const void* B, int
   hipError_t e = stile::launch_hyb_spmm(plan->spmm, b_dtype, B, ldb, N, C, ldc, det,
                                         static_cast<float*>(workspace), (hipStream_t)stream);
   return e == hipSuccess ? STILE_OK : hip_fail(e, "hyb_spmm launch");
+}
+
+int stile_spmm_run_noalloc(const StilePlan* plan, int32_t b_dtype, const void* B, int64_t ldb, int32_t N, float* C,
+                           int64_t ldc, int32_t flags, void* stream) {
+  if (!plan || plan->kind != 1) return fail(STILE_E_INVALID, "not an SpMM plan");
+  if (flags & ~STILE_RUN_DETERMINISTIC) return fail(STILE_E_INVALID, "unknown run flags");
+  if (N < 0) return fail(STILE_E_INVALID, "negative N");
+  const int64_t need =
+      (flags & STILE_RUN_DETERMINISTIC) ? stile::spmm_workspace_floats(plan->spmm, N) * 4 : 0;
+  if (need > (int64_t)STILE_NOALLOC_SCRATCH_BYTES)
+    return fail(STILE_E_INVALID, "deterministic workspace of " + std::to_string(need) +
+                                     " bytes exceeds the static scratch of STILE_NOALLOC_SCRATCH_BYTES = " +
+                                     std::to_string((int64_t)STILE_NOALLOC_SCRATCH_BYTES) + " bytes");
+  float* ws = nullptr;
+  if (need > 0) {
+    const int rc = device_scratch(&ws);
+    if (rc != STILE_OK) return rc;
+  }
+  return stile_spmm_run_ex(plan, b_dtype, B, ldb, N, C, ldc, flags, ws, need, stream);
 }
 
 int stile_spmm_run(const StilePlan* plan, const float* B, int64_t ldb, int32_t N, float* C, int64_t ldc,

@@ -81,6 +81,8 @@ def test_abi_rejects_bad_arguments_without_a_device():
     assert lib.stile_plan_create_spmm(ctypes.byref(pack), 7, ctypes.byref(out)) == 1
     assert b"blk_dtype" in lib.stile_last_error()
     assert lib.stile_spmm_run(None, None, 0, 4, None, 0, None) == 1
+    assert lib.stile_spmm_run_noalloc(None, 0, None, 4, 4, None, 4, 1, None) == 1
+    assert b"not an SpMM plan" in lib.stile_last_error()
     assert lib.stile_csr_spmm_f32(-1, 1, 1, None, None, None, None, 1, None, 1, None) == 1
     assert lib.stile_plan_destroy(None) == 0
     info = _native.StilePlanInfo()
