@@ -1007,8 +1007,8 @@ __device__ __forceinline__ void sparse_round(const TB* __restrict__ B, int64_t l
 // windows) or per workgroup (an empty unit); the caller's barrier before the
 // next item's staging keeps the LDS block until every wave is done with it.
 // kMixed: windows may have sparse parts, added (in rounds, p.sp_ent) to the
-// dense sums before the window's rows are stored; round 0 of a window is
-// requested one window ahead, the others four at a time when it ends.
+// dense sums before the window's rows are stored; rounds 0-3 of a window are
+// requested when it starts, later ones four at a time when it ends.
 template <typename TB, bool kMixed>
 __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __restrict__ tc_ptr,
                                          const int32_t* __restrict__ sp_off, const int4* __restrict__ units,
@@ -1060,8 +1060,19 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   int w_next = 0;
   SlotRows wr{};
   TrChunk r0{}, r1{};
-  int so0 = 0, so1 = 0;  // sp_off of the current window and the next (kMixed)
-  int2 e0 = make_int2(0, 0);
+  // kMixed: sp_off of the current window and the next, and the lane's
+  // entries of the current window's rounds 0 .. kPfRounds - 1, requested when
+  // the window starts and used when it ends.  The loads are unconditional
+  // (indices clamped to the last entry; a plan with mixed windows has at
+  // least one round), so the wait before their use is an exact count.
+  constexpr int kPfRounds = 4;
+  int so0 = 0, so1 = 0;
+  const int sp_last = kMixed ? sp_off[p.W_tc] - 1 : 0;
+  int2 ecur[kPfRounds];
+  auto request_rounds = [&](int so) {
+#pragma unroll
+    for (int u = 0; u < kPfRounds; ++u) ecur[u] = p.sp_ent[min(so + 16 * u + eidx, sp_last)];
+  };
   if (has) {
     w_next = tc_ptr[w + 1];
     wr = load_win(p, w * 16 + g * 4);
@@ -1070,7 +1081,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
     if constexpr (kMixed) {
       so0 = sp_off[w];
       so1 = sp_off[w + 1];
-      if (so1 > so0) e0 = p.sp_ent[so0 + eidx];
+      request_rounds(so0);
     }
   }
   stage_b(lds, B, ldb, koff, ksz, n0, N, vec, vec16);
@@ -1083,10 +1094,12 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
     if constexpr (kMixed) {
       const int rounds = (so1 - so0) >> 4;  // wave-uniform
       if (rounds > 0) {
-        sparse_round(B, ldb, N, col, vec, lds, koff, e0, acc);
-        // the other rounds' entries requested four at a time: one memory
+#pragma unroll
+        for (int u = 0; u < kPfRounds; ++u)
+          if (u < rounds) sparse_round(B, ldb, N, col, vec, lds, koff, ecur[u], acc);
+        // rounds beyond those: entries requested four at a time, one memory
         // latency per four rounds
-        for (int r = 1; r < rounds; r += 4) {
+        for (int r = kPfRounds; r < rounds; r += 4) {
           int2 e[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u)
@@ -1107,11 +1120,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
       const SlotRows wn = load_win(p, (w + 1) * 16 + g * 4);
       const int w_next2 = tc_ptr[w + 2];
       int so2 = 0;
-      int2 e0n = make_int2(0, 0);
-      if constexpr (kMixed) {
-        so2 = sp_off[w + 2];
-        if (so2 > so1) e0n = p.sp_ent[so1 + eidx];
-      }
+      if constexpr (kMixed) so2 = sp_off[w + 2];
       finish(wr);
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -1121,7 +1130,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
       if constexpr (kMixed) {
         so0 = so1;
         so1 = so2;
-        e0 = e0n;
+        request_rounds(so0);
       }
     }
     tr_chunk_mfma(lds, cst, use, acc);
