@@ -18,12 +18,20 @@ J = 32), i.e. the same unit as the A100 tables (per-SM latency of a tile).
 
   TC (16, 32, K), K = 16..160: W windows, window w holding one dense 16 x K
       block on its own K columns; N = 64.  Measured for FP32 blocks (dsize
-      32) and BF16 blocks (dsize 16), and for BF16 blocks on random columns
-      of a B of LDS_MAX_K rows (the B-stationary kernel's case).
+      32) and BF16 blocks (dsize 16) on the streaming kernel.
   ELL (I, 32, K) for the six shapes with I*K = 256: rows of exactly K
       nonzeros on uniformly random columns of a 65536-column B (and, to
       spread the feature, columns drawn from a 512-column pool); the fit
-      cost = a1 * x + b1 uses the cost model's own feature x.
+      cost = a1 * x + b1 + c1 * I uses the cost model's own feature x plus
+      a per-row term (c1 = 0 gives the reference's form).
+  B-stationary kernel (BF16 blocks, B of at most LDS_MAX_K rows), in its
+      own work-unit shape: W windows over one B of 1024 rows, each with a
+      dense block of 32 + K random columns, and with a 32-column block plus
+      r sparse entries per row (r = 1..16).  The store of a window's rows
+      and the staging of B are paid whatever a window's tiles are, so the
+      table tc_cost_lds holds what K columns add to a window (a line in K
+      fitted to the ten added costs), and the window's sparse part is
+      priced per entry plus per window (ell_lds_coeff).  B is BF16.
   1D SDDMM buckets of 32 nonzeros: rows of r nonzeros, r = 1..32, K_feat =
       32; fit of the cost model's 1D feature.
   TC SDDMM (16, J, 16): dense 16 x 16 output blocks at K_feat = J.
@@ -56,16 +64,11 @@ BF16_TFLOPS = 2500.0
 # Synthetic single-kind packs.
 # ---------------------------------------------------------------------------
 
-def tc_pack(W, K, b_rows=None, rng=None):
+def tc_pack(W, K):
     """W windows; window w: rows 16w..16w+15, one dense 16 x K tile on
-    columns w*K .. w*K+K-1 (the reference's fake TC graph), or, with b_rows,
-    on K random distinct columns of a B of b_rows rows."""
-    if b_rows is None:
-        cols = np.arange(W * K, dtype=np.int32).reshape(W, K)
-        ncols = W * K
-    else:
-        cols = np.sort(np.argsort(rng.random((W, b_rows)), axis=1)[:, :K], axis=1).astype(np.int32)
-        ncols = b_rows
+    columns w*K .. w*K+K-1 (the reference's fake TC graph)."""
+    cols = np.arange(W * K, dtype=np.int32).reshape(W, K)
+    ncols = W * K
     g = dict(group_size=K, num_windows=W, nnzb=W, rows_2d=False, k_notsorted=False, real_atomic=False,
              a=np.ones((W, 16, K), np.float32), i_indices=np.arange(W, dtype=np.int32),
              indptr=np.arange(W + 1, dtype=np.int32), indices=cols, a_ids=None)
@@ -101,6 +104,40 @@ def ell_pack(R, I, K, ncols, pool, rng):
     g = dict(m_blk=I, nnz_cols=K, num_tiles=T, real_atomic=False, a=np.ones((T, I * K), np.float32),
              indices_k=cols, indices_reordered_i=np.arange(R, dtype=np.int32), a_ids=None)
     return dict(M=R, K=ncols, ell=[g], tc=[]), cols
+
+
+def _distinct_rows(n, k, hi, rng):
+    """n rows of k distinct integers in [0, hi), each row sorted."""
+    c = np.sort(rng.integers(0, hi, size=(n, k)), axis=1)
+    while k > 1:
+        dup = np.zeros_like(c, bool)
+        dup[:, 1:] = c[:, 1:] == c[:, :-1]
+        if not dup.any():
+            break
+        c[dup] = rng.integers(0, hi, size=int(dup.sum()))
+        c.sort(axis=1)
+    return c
+
+
+def lds_pack(W, Kd, r, rng, b_rows):
+    """A B-stationary work unit shape: W windows over a B of b_rows rows;
+    window w holds one dense 16 x Kd block on Kd random distinct columns and,
+    with r > 0, r more entries in each of its rows on random columns outside
+    the block (one ELL tile of 16 rows and r columns per window, which the
+    plan turns into the window's sparse part)."""
+    perm = np.argsort(rng.random((W, b_rows)), axis=1).astype(np.int32)
+    cols = np.sort(perm[:, :Kd], axis=1)
+    g = dict(group_size=Kd, num_windows=W, nnzb=W, rows_2d=False, k_notsorted=False, real_atomic=False,
+             a=np.ones((W, 16, Kd), np.float32), i_indices=np.arange(W, dtype=np.int32),
+             indptr=np.arange(W + 1, dtype=np.int32), indices=cols, a_ids=None)
+    ell = []
+    if r:
+        idx = _distinct_rows(16 * W, r, b_rows - Kd, rng)
+        rest = perm[:, Kd:]
+        ecols = rest[np.repeat(np.arange(W), 16)[:, None], idx].astype(np.int32)
+        ell = [dict(m_blk=16, nnz_cols=r, num_tiles=W, real_atomic=False, a=np.ones((W, 16 * r), np.float32),
+                    indices_k=ecols, indices_reordered_i=np.arange(16 * W, dtype=np.int32), a_ids=None)]
+    return dict(M=16 * W, K=b_rows, ell=ell, tc=[g])
 
 
 def sddmm_1d_pack(R, r, ncols, rng):
@@ -145,10 +182,10 @@ def _time_ms(fn, reps=30, warmup=5):
     return s.elapsed_time(e) / reps
 
 
-def measure_tc(K, blk, W=8192, N=64, b_rows=None, rng=None):
+def measure_tc(K, blk, W=8192, N=64):
     import torch
     from . import _native
-    pack = tc_pack(W, K, b_rows, rng)
+    pack = tc_pack(W, K)
     plan = _native.Plan.spmm(pack, blk)
     B = torch.randn(pack["K"], N, device="cuda")
     C = torch.empty(16 * W, N, device="cuda")
@@ -168,6 +205,20 @@ def measure_ell(I, K, pool, rng, R=1 << 18, ncols=1 << 16, N=64):
     plan.close()
     T = pack["ell"][0]["num_tiles"]
     return ms * CU_NUM / (T * N / 32), cols
+
+
+def measure_lds(Kd, r, rng, W=16384, N=64, b_rows=1024, reps=100, rounds=3):
+    """Per-window cost (ms x CUs per window per 32 columns) of lds_pack(W,
+    Kd, r) on the B-stationary kernel with BF16 B: the median of `rounds`
+    timings of `reps` launches."""
+    import torch
+    from . import _native
+    plan = _native.Plan.spmm(lds_pack(W, Kd, r, rng, b_rows), "bfloat16")
+    B = torch.randn(b_rows, N, device="cuda").to(torch.bfloat16)
+    C = torch.empty(16 * W, N, device="cuda")
+    ms = float(np.median([_time_ms(lambda: plan.run_spmm(B, C), reps) for _ in range(rounds)]))
+    plan.close()
+    return ms * CU_NUM / (W * N / 32)
 
 
 def measure_sddmm_1d(r, rng, R=1 << 18, ncols=1 << 16, Kf=32):
@@ -234,6 +285,24 @@ def linear_fit(x, y):
     return cands[int(np.argmin(err))]
 
 
+def nonneg_fit(features, y, relative=True):
+    """Least-squares y = sum_i c_i f_i with every c_i >= 0.  relative: in
+    relative error (each sample weighted by 1 / y, so that small tiles count
+    as much as big ones; every y must be positive); otherwise in absolute
+    error (for added costs, which can be near 0 or measure slightly
+    negative).  features: list of arrays (one per coefficient)."""
+    from scipy.optimize import nnls
+    y = np.asarray(y, np.float64)
+    A = np.stack([np.asarray(f, np.float64) for f in features], axis=1)
+    if relative:
+        if np.any(y <= 0):
+            raise ValueError("a relative fit needs positive samples")
+        c, _ = nnls(A / y[:, None], np.ones_like(y))
+    else:
+        c, _ = nnls(A, y)
+    return [float(v) for v in c]
+
+
 def run(out_path=None, log=print):
     import torch
     from . import my_cost_model as mc
@@ -242,14 +311,12 @@ def run(out_path=None, log=print):
     peak32 = FP32_TFLOPS / HBM_TBS
     peak16 = BF16_TFLOPS / HBM_TBS
     t0 = time.time()
-    tc32, tc16, tclds = {}, {}, {}
+    tc32, tc16 = {}, {}
     for i in range(10, 0, -1):
         K = 16 * i
         tc32[(16, 32, K)] = measure_tc(K, "float32")
         tc16[(16, 32, K)] = measure_tc(K, "bfloat16")
-        tclds[(16, 32, K)] = measure_tc(K, "bfloat16", b_rows=mc.LDS_MAX_K, rng=rng)
-        log(f"TC (16,32,{K}): f32 {tc32[(16, 32, K)]:.3e}  bf16 {tc16[(16, 32, K)]:.3e}  "
-            f"bf16 B-stationary {tclds[(16, 32, K)]:.3e} ms/tile/CU")
+        log(f"TC (16,32,{K}): f32 {tc32[(16, 32, K)]:.3e}  bf16 {tc16[(16, 32, K)]:.3e} ms/tile/CU")
     ell_shapes = [(256, 1), (128, 2), (64, 4), (32, 8), (16, 16), (8, 32)]
     ell_cost, samples = {}, []
     for I, K in ell_shapes:
@@ -259,16 +326,36 @@ def run(out_path=None, log=print):
             if pool == 1 << 16:
                 ell_cost[(I, 32, K)] = c
             log(f"ELL ({I},32,{K}) pool {pool}: {c:.3e} ms/tile/CU")
-    ell_coeff, raw = {}, {"ell": [], "sddmm_1d": []}
+    ell_coeff, ell_row, raw = {}, {}, {"ell": [], "sddmm_1d": [], "lds_tc": [], "lds_ell": []}
     for dsize, peak in ((16, peak16), (32, peak32)):
-        xs, ys = [], []
+        xs, ys, Is = [], [], []
         for I, K, cols, c in samples:
             x = ell_features(I, K, cols, dsize, peak)
             xs.append(float(np.mean(x)))
             ys.append(c)
+            Is.append(I)
             raw["ell"].append({"I": I, "K": K, "dsize": dsize, "x": xs[-1], "cost": c})
-        ell_coeff[dsize] = linear_fit(np.asarray(xs), np.asarray(ys))
-        log(f"ELL fit dsize {dsize}: a1={ell_coeff[dsize][0]:.4e} b1={ell_coeff[dsize][1]:.4e}")
+        # a1 x + b1 + c1 I: these kernels pay per row of a tile (its row
+        # bounds, one store of its 32 columns) beyond what x counts
+        a1, b1, c1 = nonneg_fit([xs, np.ones(len(xs)), Is], ys)
+        ell_coeff[dsize], ell_row[dsize] = (a1, b1), c1
+        log(f"ELL fit dsize {dsize}: a1={a1:.4e} b1={b1:.4e} c1={c1:.4e}")
+    # B-stationary kernel (BF16 blocks, K <= LDS_MAX_K): the store of a
+    # window's rows and the staging of B are paid whatever its tiles are, so
+    # a TC tile is priced by what its columns add to a window that already
+    # has one full chunk, and a window's sparse part by what its rounds add.
+    base = measure_lds(32, 0, rng)
+    ks = [16 * i for i in range(1, 11)]
+    add = [measure_lds(32 + K, 0, rng) - base for K in ks]
+    alpha, beta = nonneg_fit([ks, np.ones(len(ks))], add, relative=False)
+    tclds = {(16, 32, K): alpha * K + beta for K in ks}
+    raw["lds_tc"] = [{"K": K, "added_cost": float(a)} for K, a in zip(ks, add)]
+    log(f"B-stationary TC: base window {base:.3e}, added per column {alpha:.3e} + per tile {beta:.3e}")
+    rs = [1, 2, 4, 8, 16]
+    sp = [measure_lds(32, r, rng) - base for r in rs]
+    a_e, b_w = nonneg_fit([[16 * r for r in rs], np.ones(len(rs))], sp, relative=False)
+    raw["lds_ell"] = [{"r": r, "added_cost": float(s)} for r, s in zip(rs, sp)]
+    log(f"B-stationary sparse part: per entry {a_e:.3e}, per window {b_w:.3e}")
     sd_samples = []
     for r in (1, 2, 4, 8, 16, 32):
         c, rows, cols = measure_sddmm_1d(r, rng)
@@ -290,7 +377,8 @@ def run(out_path=None, log=print):
     prof = mc.HardwareProfile(
         name="mi355x", sm_num=CU_NUM, threads_per_sm=THREADS_PER_CU,
         peak_fp32=peak32, peak_fp16=peak16, peak_tc=peak16,
-        ell_coeff=ell_coeff, ell_atomic_coeff={16: (0.0, 1.0), 32: (0.0, 1.0)},
+        ell_coeff=ell_coeff, ell_atomic_coeff={16: (0.0, 1.0), 32: (0.0, 1.0)}, ell_row_coeff=ell_row,
+        ell_lds_coeff=(a_e, b_w),
         sddmm_1d_coeff=sd_coeff, tc_cost=tc16, tc_cost_atomic=dict(tc16),
         ell_cost=ell_cost, ell_cost_atomic=dict(ell_cost), tc_sddmm_cost=tc_sd,
         tc_cost_f32=tc32, tc_cost_lds=tclds)

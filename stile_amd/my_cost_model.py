@@ -12,8 +12,15 @@ Same names and formulas as the reference module MY_sparse/my_cost_model.py:
 
 Model: roofline throughput R = min(FLOPs / bytes, PEAK) and latency
 g = a * FLOPs / R + b, with a multiplier (a2 * x_atomic / x + b2) for atomic
-ELL tiles; dense (TC) tiles use measured per-tile tables.  All constants live
-in a HardwareProfile:
+ELL tiles; dense (TC) tiles use measured per-tile tables.  A measured
+profile may add two terms the reference's kernels did not need:
+  * a per-row term c1 * I in the ELL latency (ell_row_coeff);
+  * for BF16 blocks over a B of at most LDS_MAX_K rows (the B-stationary
+    kernel, where ELL entries of rows in dense windows become those windows'
+    sparse parts, added in rounds of one entry per row), ELL tiles priced
+    per entry as one round each plus per 16 rows (ell_lds_coeff), and TC
+    tiles by what their columns add to a window (tc_cost_lds).
+All constants live in a HardwareProfile:
   'a100'   the reference's constants, measured on an A100 in FP16
            (my_cost_model.py:672-761, 786-800, 1238-1245; PEAK ratios
            19.49/1.555, 78/1.555, 312/1.555; SM_num 108)
@@ -51,6 +58,8 @@ class HardwareProfile:
     tc_sddmm_cost: dict
     tc_cost_f32: dict = None   # TC table for dsize 32 (FP32 blocks); tc_cost serves both if None
     tc_cost_lds: dict = None   # TC table for BF16 blocks when B has <= LDS_MAX_K rows (B-stationary kernel)
+    ell_row_coeff: dict = None  # dsize -> c1, per-row term of the ELL latency (0 in the reference's form)
+    ell_lds_coeff: tuple = None  # (per entry, per window) cost of dense windows' sparse parts, B-stationary kernel
 
 
 # B-stationary SpMM kernel: BF16 blocks and at most this many rows of B
@@ -115,6 +124,8 @@ def load_profile_file(path=PROFILE_PATH):
     prof = HardwareProfile(
         tc_cost_f32=conv(d["tc_cost_f32"]) if d.get("tc_cost_f32") else None,
         tc_cost_lds=conv(d["tc_cost_lds"]) if d.get("tc_cost_lds") else None,
+        ell_row_coeff={int(k): float(v) for k, v in d["ell_row_coeff"].items()} if d.get("ell_row_coeff") else None,
+        ell_lds_coeff=tuple(d["ell_lds_coeff"]) if d.get("ell_lds_coeff") else None,
         name=d["name"], sm_num=int(d["sm_num"]), threads_per_sm=int(d["threads_per_sm"]),
         peak_fp32=float(d["peak_fp32"]), peak_fp16=float(d["peak_fp16"]), peak_tc=float(d["peak_tc"]),
         ell_coeff={int(k): tuple(v) for k, v in d["ell_coeff"].items()},
@@ -130,7 +141,10 @@ def load_profile_file(path=PROFILE_PATH):
 def save_profile_file(prof, path=PROFILE_PATH, meta=None):
     enc = lambda tab: {str(tuple(k)): v for k, v in tab.items()}
     d = dict(meta=meta or {}, tc_cost_f32=enc(prof.tc_cost_f32) if prof.tc_cost_f32 else None,
-             tc_cost_lds=enc(prof.tc_cost_lds) if prof.tc_cost_lds else None, name=prof.name, sm_num=prof.sm_num, threads_per_sm=prof.threads_per_sm,
+             tc_cost_lds=enc(prof.tc_cost_lds) if prof.tc_cost_lds else None,
+             ell_row_coeff={str(k): v for k, v in prof.ell_row_coeff.items()} if prof.ell_row_coeff else None,
+             ell_lds_coeff=list(prof.ell_lds_coeff) if prof.ell_lds_coeff else None,
+             name=prof.name, sm_num=prof.sm_num, threads_per_sm=prof.threads_per_sm,
              peak_fp32=prof.peak_fp32, peak_fp16=prof.peak_fp16, peak_tc=prof.peak_tc,
              ell_coeff={str(k): list(v) for k, v in prof.ell_coeff.items()},
              ell_atomic_coeff={str(k): list(v) for k, v in prof.ell_atomic_coeff.items()},
@@ -230,18 +244,50 @@ def ELL_atomic_extra_coeff(dsize, profile=None):
     return list(p.ell_atomic_coeff[16 if dsize == 16 else 32])
 
 
-def ELL_cost_model(x_nonatomic, x_atomic, is_atomic, dsize, profile=None):
+def ELL_row_coeff(dsize, profile=None):
+    p = get_profile(profile)
+    return p.ell_row_coeff[16 if dsize == 16 else 32] if p.ell_row_coeff else 0.0
+
+
+def ELL_cost_model(x_nonatomic, x_atomic, is_atomic, dsize, profile=None, rows=0):
+    """a1 x + b1 (+ c1 rows for a profile with a per-row term), times the
+    atomic multiplier for atomic tiles."""
     a1, b1 = ELL_non_atomic_coeff(dsize, profile)
     a2, b2 = ELL_atomic_extra_coeff(dsize, profile)
+    base = a1 * x_nonatomic + b1 + ELL_row_coeff(dsize, profile) * rows
     if is_atomic:
-        return (a1 * x_nonatomic + b1) * (a2 * x_atomic / x_nonatomic + b2)
-    return a1 * x_nonatomic + b1
+        return base * (a2 * x_atomic / x_nonatomic + b2)
+    return base
+
+
+def _b_stationary(prof, dsize, op):
+    """Whether ELL entries of this op land in dense windows' sparse parts of
+    the B-stationary kernel (BF16 blocks, B of at most LDS_MAX_K rows) and
+    the profile prices those."""
+    return (prof.ell_lds_coeff is not None and dsize == 16 and op is not None
+            and getattr(op, "op_type", "spmm") == "spmm" and op.idx_lens[2] <= LDS_MAX_K)
+
+
+def lds_ell_cost(cnt, I, prof):
+    """Cost of ELL tiles of I rows holding cnt entries when they become
+    sparse parts of B-stationary windows.  The kernel adds a window's sparse
+    part in rounds, one entry of each of its 16 rows per round, at the cost
+    of 16 entries whatever the round holds; the rows of an ELL tile lie in
+    different windows, so each entry is priced as a round of its own (16 x
+    the per-entry cost measured with full rounds), plus, per 16 rows, the
+    window's sparse-part overhead."""
+    a, b = prof.ell_lds_coeff
+    return 16.0 * a * np.asarray(cnt, np.float64) + b * I / 16.0
 
 
 def cost_tb_latency_given_ELL_tiles(op, tile_sizes, tile_pos_is, dsize, ori_tot_nnz, SM_num, is_atomic,
                                     profile=None):
     """Predicted latency of ELL tiles at positions tile_pos_is (reference :1176-1214)."""
     prof = get_profile(profile)
+    I = tile_sizes[0][1]
+    if _b_stationary(prof, dsize, op):
+        cnt, _, _ = _tile_unique_cols(op, I, np.asarray(tile_pos_is, dtype=np.int64))
+        return lds_ell_cost(np.minimum(cnt, I * tile_sizes[2][1]), I, prof)
     compute_cost = compute_cost_tb_impl_given_tile_sizes("sparse_template_ell", tile_sizes)
     mem = memory_cost_tb_given_tile_sizes_and_poses(tile_sizes, tile_pos_is, "sparse_template_ell", dsize, op)
     if not is_atomic:
@@ -249,8 +295,8 @@ def cost_tb_latency_given_ELL_tiles(op, tile_sizes, tile_pos_is, dsize, ori_tot_
     peak = _peak(dsize, prof)
     tp_non_atomic, tp_atomic = [np.minimum(compute_cost / (m * 32), peak) for m in mem]
     if is_atomic:
-        return ELL_cost_model(compute_cost / tp_non_atomic, compute_cost / tp_atomic, True, dsize, profile)
-    return ELL_cost_model(compute_cost / tp_non_atomic, None, False, dsize, profile)
+        return ELL_cost_model(compute_cost / tp_non_atomic, compute_cost / tp_atomic, True, dsize, profile, rows=I)
+    return ELL_cost_model(compute_cost / tp_non_atomic, None, False, dsize, profile, rows=I)
 
 
 # ---------------------------------------------------------------------------
@@ -351,8 +397,12 @@ def lower_bound_ELL_cost_BSPMM(sub_op, max_bucket_size, dsize, profile=None):
         compute_cost = I * J * K * 2
         memory_cost = math.ceil(I * K * dsize / cacheline_bit) + math.ceil(J * dsize / cacheline_bit) * (K // 2 + 1) \
             + I * math.ceil(J * dsize / cacheline_bit)
-        tp = min(compute_cost / (memory_cost * 32), _peak(dsize, prof))
-        best = min(best, ELL_cost_model(compute_cost / tp, None, False, dsize, profile) / (I * K * J))
+        if _b_stationary(prof, dsize, sub_op):
+            cost = float(lds_ell_cost(I * K, I, prof))
+        else:
+            tp = min(compute_cost / (memory_cost * 32), _peak(dsize, prof))
+            cost = ELL_cost_model(compute_cost / tp, None, False, dsize, profile, rows=I)
+        best = min(best, cost / (I * K * J))
     return best
 
 
