@@ -107,15 +107,15 @@ def test_attention_full_64_instances(cuda):
         assert _rel(out[a:e], ref) <= 1e-2, f"instance {b}"
 
 
-def test_gcn_layer_scale18(cuda):
-    """GCN layer on R-MAT scale 18 (7.9 M nonzeros; the benchmark runs scale
-    20 with the same code): forward rows and dvals on samples vs FP64, the
-    transposed product through the sum identities."""
+def _gcn_layer_check(cuda, scale, workers=None):
+    """GCN layer on R-MAT `scale`: forward rows and dvals on samples vs FP64,
+    the forward and transposed products through the sum identities over
+    every row and column (FP32 bar 1e-5)."""
     import torch
     from stile_amd.autograd import SparseOperand
-    A, H, W = datasets.gcn_graph(18)
-    print(f"full-path: GCN scale 18 nnz={A.nnz}", flush=True)
-    op = SparseOperand(A, blk_dtype="float32", device=cuda, N=128)
+    A, H, W = datasets.gcn_graph(scale)
+    print(f"full-path: GCN scale {scale} nnz={A.nnz}", flush=True)
+    op = SparseOperand(A, blk_dtype="float32", device=cuda, N=128, workers=workers)
     assert op.symmetric
     HW = (H.astype(np.float64) @ W.astype(np.float64)).astype(np.float32)
     vals = op.values.clone().requires_grad_(True)
@@ -123,12 +123,37 @@ def test_gcn_layer_scale18(cuda):
     Y = op.spmm(HWd, vals)
     dY = np.random.default_rng(2).standard_normal(Y.shape).astype(np.float32)
     Y.backward(_t(dY, cuda))
+    Yh = Y.detach().cpu().numpy()
     rows = np.random.default_rng(0).choice(A.shape[0], 2000, replace=False)
     ref = scipy.sparse.csr_matrix(A[rows], dtype=np.float64) @ HW.astype(np.float64)
-    assert _rel(Y.detach().cpu().numpy()[rows], ref) <= 1e-5
-    _sum_identities(A, HW, Y.detach().cpu().numpy(), 1e-5)
+    assert _rel(Yh[rows], ref) <= 1e-5
+    _sum_identities(A, HW, Yh, 1e-5)
     _sum_identities(A.T.tocsr(), dY, HWd.grad.cpu().numpy(), 1e-5)
     pos = np.random.default_rng(1).choice(A.nnz, 20000, replace=False)
     prow = np.searchsorted(A.indptr, pos, side="right") - 1
     dv_ref = (dY[prow].astype(np.float64) * HW[A.indices[pos]]).sum(1)
     assert _rel(vals.grad.cpu().numpy()[pos], dv_ref) <= 1e-5
+    return A, HW, rows, ref
+
+
+def test_gcn_layer_scale18(cuda):
+    """GCN layer on R-MAT scale 18 (7.9 M nonzeros)."""
+    _gcn_layer_check(cuda, 18, workers=8)
+
+
+@pytest.mark.timeout(900)
+def test_gcn_layer_scale20_full(cuda):
+    """The GCN workload at its full size: R-MAT scale 20, 32.45 M nonzeros,
+    F = 128, with the searches on at most 8 worker processes.  FP32 blocks
+    as above, then one run with BF16 blocks (bar 1e-2: sampled rows vs FP64
+    and the sum identities)."""
+    A, HW, rows, ref = _gcn_layer_check(cuda, 20, workers=8)
+    assert A.nnz == 32453462
+    print("full-path: GCN scale 20 BF16 blocks", flush=True)
+    from stile_amd import _native
+    pack = api.spmm_pack_blocked(A, api.options_for("bfloat16"), 128, workers=8)
+    plan = _native.Plan.spmm(pack, "bfloat16")
+    C = plan.run_spmm(_t(HW, cuda)).cpu().numpy()
+    assert _rel(C[rows], ref) <= 1e-2
+    _sum_identities(A, HW, C, 1e-2)
+    plan.close()
