@@ -285,6 +285,38 @@ int stile_sddmm_run(const StilePlan* plan, const float* X, int64_t ldx,
 int stile_plan_create_spmm_batched(const StileSpmmPack* packs, int32_t count,
                                    int32_t blk_dtype, StilePlan** out);
 
+/* Build options of an SpMM plan (a variant parameter of the kernels, like
+ * the launch options below: the analogue of the reference's per-format
+ * kernel parameters its benchmark scripts sweep,
+ * SPMM/SPMMbench_our_method.py:288-320). */
+typedef struct StileSpmmOptions {
+  /* B-stationary plans (BF16 blocks, every matrix K <= 1024): a window's TC
+   * columns holding at most this many nonzeros go to scatter chunks, whose
+   * k-slots carry one entry each (4 bytes of A per entry instead of 32 per
+   * column), when that shrinks the window's A bytes.  0 = none, at most 16. */
+  int32_t scatter_max_nnz;
+} StileSpmmOptions;
+
+/* stile_plan_create_spmm_batched with build options (opts may be NULL: all
+ * zero).  count = 1 builds a plan of one matrix. */
+int stile_plan_create_spmm_ex(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype,
+                              const StileSpmmOptions* opts, StilePlan** out);
+
+/* Launch options of an SpMM plan (kernel variant parameters, read by every
+ * later run of the plan; results do not depend on them):
+ *   STILE_OPT_KERNEL    0 (default) = the B-stationary kernel when the plan
+ *                       allows it, 1 = the streaming kernel (refused with
+ *                       STILE_E_UNSUPPORTED for plans with scatter chunks);
+ *   STILE_OPT_LDS_LIST  -1 (default) = the plan's work-unit list for the
+ *                       run's column-chunk count, 0..11 = that list;
+ *   STILE_OPT_LDS_XCD   1 (default) = persistent B-stationary launches give
+ *                       each XCD a contiguous range of work items, 0 = items
+ *                       round-robin over the workgroups. */
+#define STILE_OPT_KERNEL 1
+#define STILE_OPT_LDS_LIST 2
+#define STILE_OPT_LDS_XCD 3
+int stile_plan_set_option(StilePlan* plan, int32_t option, int32_t value);
+
 /* Same for SDDMM: pattern i's rows are offset by the previous M, its columns
  * by the previous N and its output ids by the previous nnz. */
 int stile_plan_create_sddmm_batched(const StileSddmmPack* packs, int32_t count,

@@ -14,6 +14,8 @@ from . import build as _build
 STILE_BLK_F32, STILE_BLK_BF16, STILE_BLK_F16 = 0, 1, 2
 STILE_RUN_DETERMINISTIC = 1
 STILE_NOALLOC_SCRATCH_BYTES = 16 << 20
+STILE_OPT_KERNEL, STILE_OPT_LDS_LIST, STILE_OPT_LDS_XCD = 1, 2, 3
+_OPTIONS = {"kernel": STILE_OPT_KERNEL, "lds_list": STILE_OPT_LDS_LIST, "lds_xcd": STILE_OPT_LDS_XCD}
 _BLK_NAMES = {"float32": STILE_BLK_F32, "fp32": STILE_BLK_F32, "f32": STILE_BLK_F32,
               "bfloat16": STILE_BLK_BF16, "bf16": STILE_BLK_BF16,
               "float16": STILE_BLK_F16, "fp16": STILE_BLK_F16, "f16": STILE_BLK_F16}
@@ -67,6 +69,10 @@ class StilePlanInfo(ctypes.Structure):
                 ("num_split", ctypes.c_int32)]
 
 
+class StileSpmmOptions(ctypes.Structure):
+    _fields_ = [("scatter_max_nnz", ctypes.c_int32)]
+
+
 # Every symbol of include/stile.h with its ctypes signature.
 class StileEllBucket(ctypes.Structure):
     _fields_ = [("rows", ctypes.c_int32), ("width", ctypes.c_int32), ("row", ctypes.c_void_p),
@@ -88,6 +94,9 @@ _SIGNATURES = {
                                               ctypes.POINTER(ctypes.c_void_p)]),
     "stile_plan_create_spmm_batched": (ctypes.c_int, [ctypes.POINTER(StileSpmmPack), ctypes.c_int32,
                                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "stile_plan_create_spmm_ex": (ctypes.c_int, [ctypes.POINTER(StileSpmmPack), ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.POINTER(StileSpmmOptions), ctypes.POINTER(ctypes.c_void_p)]),
+    "stile_plan_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "stile_plan_create_sddmm": (ctypes.c_int, [ctypes.POINTER(StileSddmmPack), ctypes.c_int32,
                                                ctypes.POINTER(ctypes.c_void_p)]),
     "stile_plan_create_sddmm_batched": (ctypes.c_int, [ctypes.POINTER(StileSddmmPack), ctypes.c_int32,
@@ -377,17 +386,30 @@ class Plan:
         self.M, self.K, self.nnz = M, K, nnz
 
     @classmethod
-    def spmm(cls, packs, blk_dtype="bfloat16"):
-        """Build from one pack dict or a list of them (block-diagonal batch)."""
+    def spmm(cls, packs, blk_dtype="bfloat16", scatter_max_nnz=0):
+        """Build from one pack dict or a list of them (block-diagonal batch).
+        scatter_max_nnz: B-stationary plans put a window's TC columns with at
+        most that many nonzeros into scatter chunks (0 = none)."""
         lib = load()
         packs = packs if isinstance(packs, (list, tuple)) else [packs]
         arrays = _PackArrays()
         structs = (StileSpmmPack * len(packs))(*[_spmm_pack_struct(p, arrays) for p in packs])
         h = ctypes.c_void_p()
         _mark_hip()
-        _check(lib.stile_plan_create_spmm_batched(structs, len(packs), blk_dtype_code(blk_dtype),
-                                                  ctypes.byref(h)))
-        return cls(h, "spmm", sum(p["M"] for p in packs), sum(p["K"] for p in packs))
+        opts = StileSpmmOptions(scatter_max_nnz=int(scatter_max_nnz))
+        _check(lib.stile_plan_create_spmm_ex(structs, len(packs), blk_dtype_code(blk_dtype), ctypes.byref(opts),
+                                             ctypes.byref(h)))
+        plan = cls(h, "spmm", sum(p["M"] for p in packs), sum(p["K"] for p in packs))
+        plan.scatter_max_nnz = int(scatter_max_nnz)
+        return plan
+
+    def set_option(self, name, value):
+        """Launch option of an SpMM plan: 'kernel' (0 B-stationary when the
+        plan allows, 1 streaming), 'lds_list' (-1 the plan's pick, 0..11),
+        'lds_xcd' (1 per-XCD item ranges, 0 round-robin)."""
+        if name not in _OPTIONS:
+            raise StileError(f"unknown option {name!r}")
+        _check(load().stile_plan_set_option(self._h, _OPTIONS[name], int(value)))
 
     @classmethod
     def attention(cls, packs):

@@ -352,11 +352,12 @@ class StalePackError(ValueError):
     """A saved pack was built for a different matrix than the one given."""
 
 
-def save_pack(path, pack, fingerprint=None):
+def save_pack(path, pack, fingerprint=None, build=None):
     """Write an SpMM or SDDMM pack (the dicts of spmm_pack / sddmm_pack /
     the blocked variants), or a list of them (a batched plan's packs), to an
     .npz file: array fields as arrays, scalar fields and group counts as a
-    JSON header, with the matrix fingerprint (matrix_fingerprint) if given."""
+    JSON header, with the matrix fingerprint (matrix_fingerprint) and the
+    plan build options (e.g. {"scatter_max_nnz": 2}) if given."""
     import json
     arrays = {}
     if isinstance(pack, (list, tuple)):
@@ -365,14 +366,17 @@ def save_pack(path, pack, fingerprint=None):
         meta = _pack_to_arrays(pack, "", arrays)
     if fingerprint is not None:
         meta["fingerprint"] = fingerprint
+    if build:
+        meta["build"] = dict(build)
     arrays["__meta__"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez(path, **arrays)
 
 
-def load_pack(path, expect=None):
+def load_pack(path, expect=None, with_build=False):
     """Read a pack (or list of packs) written by save_pack.  expect: the
     fingerprint of the matrix the caller will run the pack on; a file saved
-    with another fingerprint, or with none, raises StalePackError."""
+    with another fingerprint, or with none, raises StalePackError.
+    with_build: return (pack, build options dict) instead."""
     import json
     with np.load(path, allow_pickle=False) as z:
         meta = json.loads(bytes(z["__meta__"]).decode())
@@ -380,8 +384,10 @@ def load_pack(path, expect=None):
             got = meta.get("fingerprint")
             raise StalePackError(f"{path}: pack built for {_fp_str(got)}, not for {_fp_str(expect)}")
         if "list" in meta:
-            return [_pack_from_arrays(m, f"p{i}/", z) for i, m in enumerate(meta["list"])]
-        return _pack_from_arrays(meta, "", z)
+            pack = [_pack_from_arrays(m, f"p{i}/", z) for i, m in enumerate(meta["list"])]
+        else:
+            pack = _pack_from_arrays(meta, "", z)
+    return (pack, dict(meta.get("build") or {})) if with_build else pack
 
 
 def _fp_str(fp):
@@ -413,6 +419,17 @@ def tuning_candidates(blk_dtype, kind="spmm"):
         cands["only_tc"] = SearchOptions(dsize=dsize, only_TC=True)
     cands["only_sparse"] = SearchOptions(dsize=dsize, only_ELL=True)
     return cands
+
+
+def plan_variants(kind, blk_dtype, mats):
+    """Build options the tuner times for every searched pack: scatter chunks
+    off and on (at most 2 nonzeros per column) for SpMM plans that run on the
+    B-stationary kernel (BF16 blocks, every matrix K <= LDS_MAX_K), else only
+    the default build."""
+    if kind == "spmm" and blk_dtype in ("bfloat16", "bf16") and \
+            max(_csr_of(A).shape[1] for A in _as_list(mats)) <= my_cost_model.LDS_MAX_K:
+        return {"": {}, "+scatter2": {"scatter_max_nnz": 2}}
+    return {"": {}}
 
 
 def _pack_any(kind, mats, opts, width, workers, log=None):
@@ -506,27 +523,31 @@ TUNE_TOL = 1e-2
 
 
 def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps=20, workers=None, log=None,
-              cache=None, b_dtype="float32"):
+              cache=None, b_dtype="float32", rounds=5):
     """Build the plan of `mats` (one matrix, or a list for a batched plan)
     under each candidate search setting, check each plan's output against a
     float64 product on a sample (relative error <= TUNE_TOL, else the
     candidate is dropped), time the rest and return (fastest plan, its pack,
-    report).  kind: 'spmm' (width = N, B of type b_dtype: float32, bfloat16
+    report).  Timing: `rounds` turns over the candidates of `reps` launches
+    each, compared by median.  kind: 'spmm' (width = N, B of type b_dtype: float32, bfloat16
     or float16, the type the plan will be run with) or 'sddmm' (width =
-    K_feat, FP32 blocks).  report: {"choice": name, "ms": {name: ms per run},
-    "search_s": {name: seconds}, "err": {name: sampled relative error}}.
-    cache: an .npz path; if it exists and was saved for this very pattern
-    (matrix_fingerprint) its pack is used without search or timing (report
-    {"choice": "cached"}); a cache saved for another pattern raises
-    StalePackError.  Otherwise the chosen pack is saved there."""
+    K_feat, FP32 blocks).  Every searched pack is timed under each build
+    variant (plan_variants: scatter chunks off / on), named search + variant.
+    report: {"choice": name, "build": build options of the choice, "ms":
+    {name: ms per run}, "search_s": {search: seconds}, "err": {name: sampled
+    relative error}}.  cache: an .npz path; if it exists and was saved for
+    this very pattern (matrix_fingerprint) its pack and build options are
+    used without search or timing (report {"choice": "cached"}); a cache
+    saved for another pattern raises StalePackError.  Otherwise the chosen
+    pack is saved there with its build options."""
     import time as _time
     import torch
     from . import _native
     if kind == "sddmm":
         blk_dtype = "float32"
     fp = matrix_fingerprint(kind, mats)
-    make = (lambda pk: _native.Plan.spmm(pk, blk_dtype)) if kind == "spmm" else \
-        (lambda pk: _native.Plan.sddmm(pk, "float32"))
+    make = (lambda pk, bo: _native.Plan.spmm(pk, blk_dtype, **bo)) if kind == "spmm" else \
+        (lambda pk, bo: _native.Plan.sddmm(pk, "float32"))
     M, K = int(_offsets(mats, 0)[-1]), int(_offsets(mats, 1)[-1])
     dev = torch.device("cuda")
     g = torch.Generator(device=dev)
@@ -547,13 +568,13 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
         return sddmm_sample_error(mats, X, Y, o)
 
     if cache is not None and os.path.exists(cache):
-        pack = load_pack(cache, expect=fp)
-        plan = make(pack)
+        pack, build = load_pack(cache, expect=fp, with_build=True)
+        plan = make(pack, build)
         err = check(plan, pack)
         if not err <= TUNE_TOL:
             plan.close()
             raise StalePackError(f"{cache}: plan error {err:.3g} against float64 exceeds {TUNE_TOL}")
-        return plan, pack, {"choice": "cached", "cache": str(cache), "err": err}
+        return plan, pack, {"choice": "cached", "cache": str(cache), "build": build, "err": err}
     cands = candidates or tuning_candidates(blk_dtype, kind)
     packs, search_s = {}, {}
     for name, opts in cands.items():          # every search before the GPU is used here
@@ -562,9 +583,12 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
         search_s[name] = round(_time.perf_counter() - t0, 3)
         if log:
             log(f"tune {kind}: searched {name} in {search_s[name]} s")
-    best, best_ms, times, errs, choice = None, float("inf"), {}, {}, None
-    for name, pack in packs.items():
-        plan = make(pack)
+    variants = plan_variants(kind, blk_dtype, mats)
+    times, errs, plans, runs, origin = {}, {}, {}, {}, {}
+    for (sname, pack), (vname, build) in ((sp, v) for sp in packs.items() for v in variants.items()):
+        name = sname + vname
+        origin[name] = (sname, build)
+        plan = make(pack, build)
         err = check(plan, pack)
         errs[name] = float(f"{err:.4g}")
         if not err <= TUNE_TOL:
@@ -573,36 +597,47 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
             times[name] = None
             plan.close()
             continue
+        plans[name] = plan
         if kind == "spmm":
-            run = lambda: plan.run_spmm(Bm, out)
+            runs[name] = (lambda pl=plan: pl.run_spmm(Bm, out))
         else:
             o = torch.empty(pack_nnz(pack), device=dev)
-            run = lambda: plan.run_sddmm(X, Y, o)
-        for _ in range(3):
-            run()
-        torch.cuda.synchronize()
-        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s.record()
-        for _ in range(reps):
-            run()
-        e.record()
-        torch.cuda.synchronize()
-        ms = s.elapsed_time(e) / reps
+            runs[name] = (lambda pl=plan, o=o: pl.run_sddmm(X, Y, o))
+    # the candidates are timed in turn, `rounds` times over, and compared by
+    # their median: a drift of the clocks during tuning then shifts every
+    # candidate alike instead of favouring the one timed first
+    samples = {name: [] for name in plans}
+    for _ in range(rounds):
+        for name, run in runs.items():
+            for _ in range(3):
+                run()
+            torch.cuda.synchronize()
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record()
+            for _ in range(reps):
+                run()
+            e.record()
+            torch.cuda.synchronize()
+            samples[name].append(s.elapsed_time(e) / reps)
+    best, best_ms, choice = None, float("inf"), None
+    for name in plans:
+        ms = float(np.median(samples[name]))
         times[name] = round(ms, 5)
         if log:
-            log(f"tune {kind} {name}: {ms * 1e3:.2f} us (search {search_s[name]} s, error {err:.2e})")
+            log(f"tune {kind} {name}: {ms * 1e3:.2f} us (search {search_s[origin[name][0]]} s, "
+                f"error {errs[name]:.2e})")
         if ms < best_ms:
-            if best is not None:
-                best.close()
-            best, best_ms, choice = plan, ms, name
-        else:
+            best, best_ms, choice = plans[name], ms, name
+    for name, plan in plans.items():
+        if name != choice:
             plan.close()
     if best is None:
         raise RuntimeError(f"tune {kind}: no candidate plan passed the float64 check ({errs})")
+    sname, build = origin[choice]
     if cache is not None:
         os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
-        save_pack(cache, packs[choice], fingerprint=fp)
-    return best, packs[choice], {"choice": choice, "ms": times, "search_s": search_s, "err": errs}
+        save_pack(cache, packs[sname], fingerprint=fp, build=build)
+    return best, packs[sname], {"choice": choice, "build": build, "ms": times, "search_s": search_s, "err": errs}
 
 
 def pack_nnz(pack):

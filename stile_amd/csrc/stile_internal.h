@@ -17,6 +17,10 @@ constexpr int kHubNnz = 4096;
 // fix-up kernel adds with atomicAdd after the window pass (in deterministic
 // mode: stores to a workspace, summed in piece order by a reduce kernel).
 constexpr int kSplitNnz = 256;
+
+// Value id of an A element that a value refresh leaves as it is (the row
+// half of a scatter chunk's k-slot word).
+constexpr int32_t kKeepId = -2;
 // Windows with more than kBigChunks dense chunks are "big": the streaming
 // kernel gives each (big window, 64 columns) a workgroup of kBigWaves waves,
 // each wave a contiguous share of the chunks, and sums the partial results
@@ -100,6 +104,16 @@ struct SpmmPlanDev {
   // nullptr when the plan does not qualify for that kernel.
   const int32_t* sp_off;     // [W_tc + 1]
   const int2* sp_ent;
+  // B-stationary plans: bit c % 32 of tc_kind[c / 32] set = chunk c is a
+  // scatter chunk (k-slot k holds one entry: the 32-bit word (BF16 value) |
+  // (row >= 8) << 16 at byte 4k of the chunk's A block, the row being
+  // k % 8 or k % 8 + 8); nullptr when the plan has none.  A window's scatter
+  // chunks follow its dense chunks.
+  const uint32_t* tc_kind;
+  // launch options (stile_plan_set_option): kernel 0 = B-stationary when the
+  // plan allows it, 1 = streaming; unit list -1 = the plan's pick, else that
+  // list; XCD ranges 1 = on, 0 = off
+  int32_t opt_kernel, opt_lds_list, opt_lds_xcd;
   const int32_t* win_rows;   // [W*16]
   const int32_t* tc_ptr;     // [W+1]
   const int32_t* tc_cols;    // [nchunks*32]

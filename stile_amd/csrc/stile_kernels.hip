@@ -936,24 +936,50 @@ __device__ __forceinline__ void stage_b(lds_u8* lds, const TB* __restrict__ B, i
 }
 
 // Column ids of k-slots (g, e') and (g, e' + 4) (the rows this lane
-// addresses in the transposed reads) and the lane's A operand of one chunk.
+// addresses in the transposed reads) and the lane's A operand of one chunk
+// or, for a scatter chunk (sc, wave-uniform), the 16 bytes of k-slot words
+// that hold the lane's one entry.
 struct TrChunk {
   int k0, k1;
   bf16x8 a;
+  bool sc;
 };
 
-__device__ __forceinline__ TrChunk load_tr_chunk(const SpmmPlanDev& p, int c, int cidx, int lane) {
+__device__ __forceinline__ TrChunk load_tr_chunk(const SpmmPlanDev& p, int c, int cidx, int lane, bool sc) {
   TrChunk r;
   const int32_t* cp = p.tc_cols + (int64_t)c * 32 + cidx;
   r.k0 = cp[0];
   r.k1 = cp[4];
-  r.a = reinterpret_cast<const bf16x8*>(p.tc_vals)[(int64_t)c * 64 + lane];
+  // scatter chunk: lane (G, row) needs the word of k-slot 8G + row % 8, in
+  // 16-byte group 2G + (row % 8) / 4 of the chunk's A block
+  const int q = sc ? 2 * (lane >> 4) + ((lane >> 2) & 1) : lane;
+  r.a = reinterpret_cast<const bf16x8*>(p.tc_vals)[(int64_t)c * 64 + q];
+  r.sc = sc;
   return r;
+}
+
+// The MFMA A operand of a scatter chunk for lane (G, row): element row % 8
+// holds the value of k-slot 8G + row % 8 when that entry's row is this
+// lane's (its word's bit 16 = row >= 8), every other element 0.
+__device__ __forceinline__ bf16x8 scatter_operand(const bf16x8& raw, int lane) {
+  const uint4 w4 = __builtin_bit_cast(uint4, raw);
+  const int s = lane & 3;
+  const uint32_t v = s == 0 ? w4.x : s == 1 ? w4.y : s == 2 ? w4.z : w4.w;
+  const uint32_t val = ((v >> 16) & 1u) == (uint32_t)((lane >> 3) & 1) ? (v & 0xffffu) : 0u;
+  const uint32_t h = (lane & 1) ? val << 16 : val;
+  const int word = (lane >> 1) & 3;
+  uint4 o;
+  o.x = word == 0 ? h : 0u;
+  o.y = word == 1 ? h : 0u;
+  o.z = word == 2 ? h : 0u;
+  o.w = word == 3 ? h : 0u;
+  return __builtin_bit_cast(bf16x8, o);
 }
 
 // cst = 8q - koff * kLdsRowBytes: the lane's position offset within a row,
 // and the unit's first B row
 __device__ __forceinline__ void tr_chunk_mfma(lds_u8* lds, int cst, const TrChunk& r, f32x4 (&acc)[4]) {
+  const bf16x8 a = r.sc ? scatter_operand(r.a, threadIdx.x & 63) : r.a;
   // 24-bit multiplies (column ids < 2^24): one full-rate v_mad_u32_u24 each
   lds_u8* p0 = lds + (int)(__umul24((unsigned)r.k0, (unsigned)kLdsRowBytes) + (unsigned)cst);
   lds_u8* p1 = lds + (int)(__umul24((unsigned)r.k1, (unsigned)kLdsRowBytes) + (unsigned)cst);
@@ -962,7 +988,7 @@ __device__ __forceinline__ void tr_chunk_mfma(lds_u8* lds, int cst, const TrChun
     const i16x4 lo = lds_tr_read(p0 + 32 * t);
     const i16x4 hi = lds_tr_read(p1 + 32 * t);
     const i16x8 b = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
-    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.a, __builtin_bit_cast(bf16x8, b), acc[t], 0, 0, 0);
+    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, b), acc[t], 0, 0, 0);
   }
 }
 
@@ -1007,11 +1033,14 @@ __device__ __forceinline__ void sparse_round(const TB* __restrict__ B, int64_t l
 // windows) or per workgroup (an empty unit); the caller's barrier before the
 // next item's staging keeps the LDS block until every wave is done with it.
 // kMixed: windows may have sparse parts, added (in rounds, p.sp_ent) to the
-// dense sums before the window's rows are stored; rounds 0-3 of a window are
-// requested when it starts, later ones four at a time when it ends.
+// dense sums before the window's rows are stored; a window's first rounds
+// (four, two with FP32 B) are requested when it starts, later ones four at a
+// time when it ends.  Scatter chunks (p.tc_kind) build their A operand from
+// one k-slot word per lane.
 template <typename TB, bool kMixed>
 __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __restrict__ tc_ptr,
-                                         const int32_t* __restrict__ sp_off, const int4* __restrict__ units,
+                                         const int32_t* __restrict__ sp_off, const uint32_t* __restrict__ tc_kind,
+                                         const int4* __restrict__ units,
                                          const TB* __restrict__ B, int64_t ldb, int N, float* __restrict__ C,
                                          int64_t ldc, int nC, bool vec, bool vec16, int item, lds_u8* lds) {
   constexpr int kWaves = kLdsThreads / 64;
@@ -1056,6 +1085,8 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   // chunks' operands are requested before the B block is staged, so their
   // latency overlaps the staging.
   const int last = c_end - 1;
+  // chunk kind (scatter or dense): a scalar load of its word of tc_kind
+  auto is_sc = [&](int idx) { return tc_kind && ((tc_kind[idx >> 5] >> (idx & 31)) & 1u) != 0; };
   int w = w_beg;
   int w_next = 0;
   SlotRows wr{};
@@ -1065,7 +1096,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   // the window starts and used when it ends.  The loads are unconditional
   // (indices clamped to the last entry; a plan with mixed windows has at
   // least one round), so the wait before their use is an exact count.
-  constexpr int kPfRounds = 4;
+  constexpr int kPfRounds = std::is_same<TB, float>::value ? 2 : 4;  // FP32 B: fewer, for registers
   int so0 = 0, so1 = 0;
   const int sp_last = kMixed ? sp_off[p.W_tc] - 1 : 0;
   int2 ecur[kPfRounds];
@@ -1076,8 +1107,8 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   if (has) {
     w_next = tc_ptr[w + 1];
     wr = load_win(p, w * 16 + g * 4);
-    r0 = load_tr_chunk(p, c_beg, cidx, lane);
-    r1 = load_tr_chunk(p, min(c_beg + 1, last), cidx, lane);
+    r0 = load_tr_chunk(p, c_beg, cidx, lane, is_sc(c_beg));
+    r1 = load_tr_chunk(p, min(c_beg + 1, last), cidx, lane, is_sc(min(c_beg + 1, last)));
     if constexpr (kMixed) {
       so0 = sp_off[w];
       so1 = sp_off[w + 1];
@@ -1115,7 +1146,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   TrChunk r2;
   int c = c_beg;
   auto step = [&](const TrChunk& use, TrChunk& fill) {
-    fill = load_tr_chunk(p, min(c + 2, last), cidx, lane);
+    fill = load_tr_chunk(p, min(c + 2, last), cidx, lane, is_sc(min(c + 2, last)));
     if (c >= w_next) {  // window w is complete: store it
       const SlotRows wn = load_win(p, (w + 1) * 16 + g * 4);
       const int w_next2 = tc_ptr[w + 2];
@@ -1165,6 +1196,7 @@ template <bool kLoop, typename TB, bool kMixed>
 __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p,
                                                                         const int32_t* __restrict__ tc_ptr,
                                                                         const int32_t* __restrict__ sp_off,
+                                                                        const uint32_t* __restrict__ tc_kind,
                                                                         const int4* __restrict__ units,
                                                                         const TB* __restrict__ B,
                                                                         int64_t ldb, int N, float* __restrict__ C,
@@ -1176,7 +1208,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   const int tb = (int)gridDim.x - 1 - (int)blockIdx.x;
   if (!kLoop) {
     if ((int)blockIdx.x < n_items)
-      lds_item<TB, kMixed>(p, tc_ptr, sp_off, units, B, ldb, N, C, ldc, nC, vec, vec16, blockIdx.x, lds);
+      lds_item<TB, kMixed>(p, tc_ptr, sp_off, tc_kind, units, B, ldb, N, C, ldc, nC, vec, vec16, blockIdx.x, lds);
     if (tb < tail_blocks) {
       const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
       if (task < tail_tasks) sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, task);
@@ -1199,7 +1231,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   }
   for (int item = beg; item < end; item += step) {
     __syncthreads();  // every wave is done reading the previous item's block
-    lds_item<TB, kMixed>(p, tc_ptr, sp_off, units, B, ldb, N, C, ldc, nC, vec, vec16, item, lds);
+    lds_item<TB, kMixed>(p, tc_ptr, sp_off, tc_kind, units, B, ldb, N, C, ldc, nC, vec, vec16, item, lds);
   }
   if (tb < tail_blocks) {
     const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
@@ -1320,40 +1352,6 @@ int device_cus() {
   return n;
 }
 
-// STILE_SPMM_KERNEL=stream forces the streaming dense kernel for BF16 plans
-// the B-stationary kernel would run (for comparisons); "lds" or unset: the
-// default choice.  Read once (a function-local static is initialised once,
-// also with several host threads).
-static int spmm_kernel_choice() {
-  static const int choice = [] {
-    const char* e = getenv("STILE_SPMM_KERNEL");
-    return (e && !strcmp(e, "stream")) ? 1 : 0;
-  }();
-  return choice;
-}
-
-// STILE_LDS_XCD=0 turns the XCD grouping of persistent B-stationary launches
-// off (for comparisons); unset or 1: on.  Read once.
-static bool lds_xcd_enabled() {
-  static const bool on = [] {
-    const char* e = getenv("STILE_LDS_XCD");
-    return !(e && e[0] == '0');
-  }();
-  return on;
-}
-
-// STILE_LDS_LIST=t forces unit list t of the B-stationary kernel (for launch
-// shape sweeps); unset or out of range: the plan's choice for the column
-// chunk count.  Read once.
-static int lds_list_override() {
-  static const int t = [] {
-    const char* e = getenv("STILE_LDS_LIST");
-    const int v = e ? atoi(e) : -1;
-    return (v >= 0 && v < kLdsLists) ? v : -1;
-  }();
-  return t;
-}
-
 int64_t spmm_workspace_floats(const SpmmPlanDev& p, int N) {
   return ((int64_t)p.npieces + (int64_t)p.n_parts * 16) * N;
 }
@@ -1375,8 +1373,11 @@ static hipError_t launch_hyb_spmm_t(const SpmmPlanDev& p, const TB* B, int64_t l
   // streaming at N = 128, 20.8 vs 82 us at N = 4096); otherwise the
   // streaming kernel.  Unit list: the plan's pick for nC column chunks (up to
   // kLdsPickMax), else the smallest list with at least one item per CU.
+  // (launch options of the plan: opt_kernel 1 = streaming kernel, refused
+  // at plan level for plans with scatter chunks; opt_lds_list >= 0 = that
+  // unit list; opt_lds_xcd 0 = items round-robin instead of per-XCD ranges)
   bool lds = p.lds_units != nullptr && p.W_tc > 0;
-  if (spmm_kernel_choice() == 1) lds = false;
+  if (p.opt_kernel == 1 && !p.tc_kind) lds = false;
   // Sparse-only rows: with the LDS kernel and at most kLdsTailMaxTasks
   // (row, 64 columns) tasks they are the LDS launch's tail; otherwise a
   // launch of their own follows the dense kernels.
@@ -1388,7 +1389,7 @@ static hipError_t launch_hyb_spmm_t(const SpmmPlanDev& p, const TB* B, int64_t l
   // the streaming kernel does the dense parts (its first blocks the big
   // windows) and the sparse kernel the sparse parts.
   if (lds) {
-    int t = lds_list_override();
+    int t = p.opt_lds_list >= 0 && p.opt_lds_list < kLdsLists ? p.opt_lds_list : -1;
     if (t < 0 && nC <= kLdsPickMax) t = p.lds_pick[nC];
     if (t < 0) {
       t = 0;
@@ -1399,7 +1400,7 @@ static hipError_t launch_hyb_spmm_t(const SpmmPlanDev& p, const TB* B, int64_t l
     // one workgroup per CU at most (each holds a staged block of LDS)
     int grid = (int)std::min<int64_t>(items, cus);
     LdsXcdRanges xr{};
-    const bool xcd = items > cus && nC <= kLdsXcdMaxNC && cus >= kXcds && p.lds_xcd_host && lds_xcd_enabled();
+    const bool xcd = items > cus && nC <= kLdsXcdMaxNC && cus >= kXcds && p.lds_xcd_host && p.opt_lds_xcd != 0;
     if (xcd) {
       const int32_t* st = p.lds_xcd_host + ((size_t)t * (kLdsXcdMaxNC + 1) + nC) * (kXcds + 1);
       if (st[0] != 0 || st[kXcds] != items) return hipErrorInvalidValue;  // ranges cover every item once
@@ -1419,7 +1420,8 @@ static hipError_t launch_hyb_spmm_t(const SpmmPlanDev& p, const TB* B, int64_t l
     const bool mixed = p.n_mix > 0 && p.sp_off != nullptr;
 #define STILE_LDS_LAUNCH(LOOP, MIX)                                                                          \
   hipLaunchKernelGGL((hyb_spmm_lds_bf16_kernel<LOOP, TB, MIX>), dim3(grid), dim3(kLdsThreads), lds_bytes, s, p, \
-                     p.tc_ptr, p.sp_off, units, B, ldb, N, C, ldc, nC, (int)items, vec, vec16, xr, LOOP && xcd, \
+                     p.tc_ptr, p.sp_off, p.tc_kind, units, B, ldb, N, C, ldc, nC, (int)items, vec, vec16, xr,      \
+                     LOOP && xcd,                                                                              \
                      tail_blocks, tail_tasks)
     if (items <= cus) {
       if (mixed) STILE_LDS_LAUNCH(false, true); else STILE_LDS_LAUNCH(false, false);
@@ -1979,6 +1981,7 @@ __global__ __launch_bounds__(256) void set_values_kernel(const float* __restrict
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     if (i < n_tc) {
       const int id = tc_ids[i];
+      if (id == kKeepId) continue;
       const float v = id >= 0 ? vals[id] : 0.f;
       if (mode == 1)
         reinterpret_cast<__bf16*>(tc_vals)[i] = (__bf16)v;

@@ -122,11 +122,21 @@ struct DenseCol {
   int32_t id[16]; // value ids (index into A's value array), -1 = none
 };
 
+// One nonzero of a TC column that goes to a scatter chunk (B-stationary
+// plans): its column, window slot (row 0..15), value and value id.
+struct ScatEnt {
+  int32_t col;
+  int32_t row;
+  float v;
+  int32_t id;
+};
+
 struct SpmmBuild {
   int32_t M = 0, K = 0;
   // windows
   std::vector<int32_t> win_rows;               // [W*16]
   std::vector<std::vector<DenseCol>> win_cols;  // dense columns per window
+  std::vector<std::vector<ScatEnt>> win_scat;   // scatter-chunk entries per window
   std::vector<int32_t> row_win, row_slot;      // per output row
   // sparse part, as (row, col, val) triples
   std::vector<int32_t> s_row, s_col;
@@ -546,8 +556,70 @@ void order_chunk_slots(std::vector<DenseCol>& cols, int64_t koff) {
   }
 }
 
-int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
+// Scatter chunks (B-stationary plans): a chunk whose 32 k-slots hold one
+// nonzero each.  The MFMA A operand gives lane (G, r) the elements A[r][8G +
+// e], e = 0..7; the entry of k-slot 8G + e is placed so that its row is e or
+// e + 8, so lane (G, r) has its one possible value at element r % 8 and the
+// chunk stores per k-slot only that value and whether the row is >= 8 (4
+// bytes, against 32 for a column of a dense chunk).  Returns the number of
+// scatter chunks the entries need.
+int32_t scatter_chunks(const std::vector<ScatEnt>& s) {
+  int32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (const ScatEnt& e : s) cnt[e.row & 7]++;
+  int32_t m = 0;
+  for (int r = 0; r < 8; ++r) m = std::max(m, (cnt[r] + 3) / 4);
+  return m;
+}
+
+// k-slot contents of a window's scatter chunks: per chunk 32 indices into s
+// (-1: padding).  Chunk j takes entries 4j .. 4j + 3 of each row-residue
+// class e; within a chunk the entries of class e are placed over the four
+// lane groups G so that each half-wave's transposed B read (k-slots
+// {8G + e: G in a pair, e in a half}) meets distinct B-row residues mod 8
+// where it can (the LDS bank groups, as for dense chunks).
+std::vector<std::array<int32_t, 32>> place_scatter(const std::vector<ScatEnt>& s, int64_t koff) {
+  std::vector<int32_t> cls[8];
+  for (size_t i = 0; i < s.size(); ++i) cls[s[i].row & 7].push_back((int32_t)i);
+  const int32_t nch = scatter_chunks(s);
+  std::vector<std::array<int32_t, 32>> out(nch);
+  for (int32_t j = 0; j < nch; ++j) {
+    auto& slot = out[j];
+    slot.fill(-1);
+    for (int h = 0; h < 2; ++h) {
+      unsigned used[2] = {0, 0};  // B-row residues read by lane-group pair 0-1 / 2-3
+      for (int e = 4 * h; e < 4 * h + 4; ++e) {
+        int32_t ents[4];
+        int m = 0;
+        for (size_t q = (size_t)4 * j; q < cls[e].size() && m < 4; ++q) ents[m++] = cls[e][q];
+        // best placement of the m entries over the lane groups 0..3
+        int perm[4] = {0, 1, 2, 3}, best[4] = {0, 1, 2, 3}, best_cost = 1 << 30;
+        do {
+          unsigned u[2] = {used[0], used[1]};
+          int cost = 0;
+          for (int q = 0; q < m; ++q) {
+            const int pair = perm[q] >> 1;
+            const unsigned bit = 1u << ((s[ents[q]].col - koff) & 7);
+            if (u[pair] & bit) ++cost;
+            u[pair] |= bit;
+          }
+          if (cost < best_cost) {
+            best_cost = cost;
+            std::copy(perm, perm + 4, best);
+          }
+        } while (best_cost > 0 && std::next_permutation(perm, perm + 4));
+        for (int q = 0; q < m; ++q) {
+          slot[8 * best[q] + e] = ents[q];
+          used[best[q] >> 1] |= 1u << ((s[ents[q]].col - koff) & 7);
+        }
+      }
+    }
+  }
+  return out;
+}
+
+int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, int32_t scatter_max, StilePlan** out) {
   const int32_t W = (int32_t)(bd.win_rows.size() / 16);
+  bd.win_scat.assign(W, {});
 
   // Dense columns of each window: merge duplicates (a column covered by
   // several tiles of the window), sort by column.
@@ -569,23 +641,52 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
     }
     cols.resize(o);
   }
+  // BF16 plans whose matrices all have K <= kLdsMaxK run on the B-stationary
+  // kernel, which stages one matrix's B rows per workgroup.
+  int64_t max_k = 0;
+  for (size_t i = 0; i + 1 < bd.inst_cols.size(); ++i) max_k = std::max(max_k, bd.inst_cols[i + 1] - bd.inst_cols[i]);
+  const bool lds_ok = blk_dtype == STILE_BLK_BF16 && max_k <= stile::kLdsMaxK && !bd.want_mask;
+  // B-stationary plans: a window's TC columns with at most scatter_max
+  // nonzeros go to scatter chunks when that shrinks the window's A bytes.
+  if (lds_ok && scatter_max > 0) {
+    for (int32_t w = 0; w < W; ++w) {
+      auto& cols = bd.win_cols[w];
+      std::vector<DenseCol> keep;
+      std::vector<ScatEnt> sc;
+      for (const DenseCol& dc : cols) {
+        int n = 0;
+        for (int r = 0; r < 16; ++r) n += (dc.id[r] >= 0 || dc.v[r] != 0.f) ? 1 : 0;
+        if (n > scatter_max) {
+          keep.push_back(dc);
+        } else {
+          for (int r = 0; r < 16; ++r)
+            if (dc.id[r] >= 0 || dc.v[r] != 0.f) sc.push_back(ScatEnt{dc.col, r, dc.v[r], dc.id[r]});
+        }
+      }
+      const int64_t before = (int64_t)(cols.size() + 31) / 32 * 1024;
+      const int64_t after = (int64_t)(keep.size() + 31) / 32 * 1024 + (int64_t)scatter_chunks(sc) * 128;
+      if (!sc.empty() && after < before) {
+        cols.swap(keep);
+        bd.win_scat[w].swap(sc);
+      }
+    }
+  }
+  auto has_chunks = [&](int32_t w) { return !bd.win_cols[w].empty() || !bd.win_scat[w].empty(); };
   // Windows with more than stile::kBigChunks dense chunks first (most chunks first,
   // W_big of them), then the other windows with dense chunks (W_tc in all),
   // then the sparse-only windows; the kernels map the kinds onto waves
   // differently.
   std::vector<int32_t> order(W);
   for (int32_t w = 0; w < W; ++w) order[w] = w;
-  auto nch = [&](int32_t w) { return (int64_t)(bd.win_cols[w].size() + 31) / 32; };
-  std::stable_partition(order.begin(), order.end(), [&](int32_t w) { return !bd.win_cols[w].empty(); });
+  auto nch = [&](int32_t w) {
+    return (int64_t)(bd.win_cols[w].size() + 31) / 32 + scatter_chunks(bd.win_scat[w]);
+  };
+  std::stable_partition(order.begin(), order.end(), [&](int32_t w) { return has_chunks(w); });
   int32_t W_tc = 0;
-  while (W_tc < W && !bd.win_cols[order[W_tc]].empty()) ++W_tc;
-  // BF16 plans whose matrices all have K <= kLdsMaxK run on the B-stationary
-  // kernel, which stages one matrix's B rows per workgroup: in a batch of
-  // several matrices their dense windows are kept grouped by matrix (and no
-  // window counts as big or huge, which only the streaming kernel uses).
-  int64_t max_k = 0;
-  for (size_t i = 0; i + 1 < bd.inst_cols.size(); ++i) max_k = std::max(max_k, bd.inst_cols[i + 1] - bd.inst_cols[i]);
-  const bool lds_ok = blk_dtype == STILE_BLK_BF16 && max_k <= stile::kLdsMaxK && !bd.want_mask;
+  while (W_tc < W && has_chunks(order[W_tc])) ++W_tc;
+  // B-stationary plans of several matrices keep their dense windows grouped
+  // by matrix (and no window counts as big or huge, which only the streaming
+  // kernel uses).
   const bool by_matrix = lds_ok && bd.inst_rows.size() > 2;
   int32_t W_big = 0, W_huge = 0;
   if (by_matrix) {
@@ -604,12 +705,15 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
     for (int32_t nw = 0; nw < W; ++nw) new_of[order[nw]] = nw;
     std::vector<int32_t> rows2(bd.win_rows.size());
     std::vector<std::vector<DenseCol>> cols2(W);
+    std::vector<std::vector<ScatEnt>> scat2(W);
     for (int32_t nw = 0; nw < W; ++nw) {
       for (int r = 0; r < 16; ++r) rows2[(size_t)nw * 16 + r] = bd.win_rows[(size_t)order[nw] * 16 + r];
       cols2[nw].swap(bd.win_cols[order[nw]]);
+      scat2[nw].swap(bd.win_scat[order[nw]]);
     }
     bd.win_rows.swap(rows2);
     bd.win_cols.swap(cols2);
+    bd.win_scat.swap(scat2);
     for (auto& rw : bd.row_win)
       if (rw >= 0) rw = new_of[rw];
   }
@@ -681,11 +785,16 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   if (lds_ok)
     for (int32_t w = 0; w < W_tc; ++w) order_chunk_slots(bd.win_cols[w], bd.inst_cols[window_matrix(bd, w)]);
   // Dense chunks: cut each window's columns into chunks of 32 k-slots (the
-  // last one padded with value-0 slots that point at the chunk's first column).
-  std::vector<int32_t> tc_ptr(W + 1, 0);
+  // last one padded with value-0 slots that point at the chunk's first
+  // column), then the window's scatter chunks (tc_kind bit set).
+  std::vector<int32_t> tc_ptr(W + 1, 0), n_dense(W, 0);
   int64_t nchunks = 0;
+  bool any_scatter = false;
   for (int32_t w = 0; w < W; ++w) {
-    nchunks += (int64_t)(bd.win_cols[w].size() + 31) / 32;
+    n_dense[w] = (int32_t)((bd.win_cols[w].size() + 31) / 32);
+    const int32_t ns = scatter_chunks(bd.win_scat[w]);
+    any_scatter = any_scatter || ns > 0;
+    nchunks += n_dense[w] + ns;
     tc_ptr[w + 1] = (int32_t)nchunks;
   }
   if (nchunks > INT32_MAX / 64) return fail(STILE_E_UNSUPPORTED, "too many dense chunks");
@@ -742,9 +851,10 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   std::vector<int32_t> tc_ids(bd.has_ids ? (size_t)nchunks * 64 * 8 : 0);
   // tc_mask[c][r] bit s: k-slot s of chunk c holds an entry of window row r
   std::vector<uint32_t> tc_mask(bd.want_mask ? (size_t)nchunks * 16 : 0);
+  std::vector<uint32_t> tc_kind(any_scatter ? (size_t)(nchunks + 31) / 32 : 0);
   for (int32_t w = 0; w < W; ++w) {
     const auto& cols = bd.win_cols[w];
-    for (int64_t c = tc_ptr[w]; c < tc_ptr[w + 1]; ++c) {
+    for (int64_t c = tc_ptr[w]; c < tc_ptr[w] + n_dense[w]; ++c) {
       if (bd.want_mask) {
         const size_t b0 = (size_t)(c - tc_ptr[w]) * 32;
         for (int r = 0; r < 16; ++r) {
@@ -790,6 +900,56 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
         }
       }
     }
+    // Scatter chunks (BF16): k-slot k holds one entry, stored in the first
+    // 128 bytes of the chunk's A block as the 32-bit word (BF16 value) |
+    // (row >= 8) << 16 at byte 4k.  Its value id goes with the value half
+    // (A element 2k); the row half keeps its bit across value refreshes.
+    // Padding slots: value 0, column = a B row of the matrix whose residue
+    // its half-wave read does not meet yet.
+    if (!bd.win_scat[w].empty()) {
+      const auto& sc = bd.win_scat[w];
+      const int32_t m = window_matrix(bd, w);
+      const int64_t koff = bd.inst_cols[m], ksz = bd.inst_cols[m + 1] - bd.inst_cols[m];
+      const auto slots = place_scatter(sc, koff);
+      for (size_t j = 0; j < slots.size(); ++j) {
+        const int64_t c = tc_ptr[w] + n_dense[w] + (int64_t)j;
+        tc_kind[(size_t)c >> 5] |= 1u << (c & 31);
+        if (bd.has_ids) std::fill(tc_ids.begin() + c * 512, tc_ids.begin() + (c + 1) * 512, -1);
+        for (int pair = 0; pair < 4; ++pair) {  // k-slots 8G + e, G in {2(pair>>1), +1}, e in 4(pair&1) + 0..3
+          unsigned used = 0;
+          for (int G = 2 * (pair >> 1); G < 2 * (pair >> 1) + 2; ++G)
+            for (int e = 4 * (pair & 1); e < 4 * (pair & 1) + 4; ++e)
+              if (slots[j][8 * G + e] >= 0) used |= 1u << ((sc[slots[j][8 * G + e]].col - koff) & 7);
+          for (int G = 2 * (pair >> 1); G < 2 * (pair >> 1) + 2; ++G)
+            for (int e = 4 * (pair & 1); e < 4 * (pair & 1) + 4; ++e) {
+              const int k = 8 * G + e;
+              const int32_t q = slots[j][k];
+              int32_t col = (int32_t)koff;
+              uint32_t word = 0;
+              int32_t id = -1;
+              if (q >= 0) {
+                col = sc[q].col;
+                word = (uint32_t)to_bf16_bits(sc[q].v) | (sc[q].row >= 8 ? 1u << 16 : 0u);
+                id = sc[q].id;
+              } else {
+                for (int r = 0; r < 8 && r < ksz; ++r)
+                  if (!(used >> r & 1u)) {
+                    col = (int32_t)(koff + r);
+                    used |= 1u << r;
+                    break;
+                  }
+              }
+              tc_cols[(size_t)c * 32 + k] = col;
+              std::memcpy(&tc_vals[(size_t)c * 1024 + 4 * k], &word, 4);
+              if (bd.has_ids) {
+                const size_t at = ((size_t)c * 64 + k / 4) * 8 + 2 * (k % 4);
+                tc_ids[at] = id;
+                tc_ids[at + 1] = stile::kKeepId;
+              }
+            }
+        }
+      }
+    }
   }
 
   Layout lay;
@@ -819,6 +979,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   const size_t o_spoff = lay.add(sp_off.size() * 4);
   const size_t o_spent = lay.add(sp_ent.size() * 4);
   const size_t o_spids = lay.add(sp_ids.size() * 4);
+  const size_t o_kind = lay.add(tc_kind.size() * 4);
   HostBlob blob;
   blob.data.assign(lay.bytes, 0);
   blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
@@ -847,6 +1008,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   blob.put(o_spoff, sp_off.data(), sp_off.size() * 4);
   blob.put(o_spent, sp_ent.data(), sp_ent.size() * 4);
   blob.put(o_spids, sp_ids.data(), sp_ids.size() * 4);
+  blob.put(o_kind, tc_kind.data(), tc_kind.size() * 4);
 
   StilePlan* plan = new (std::nothrow) StilePlan();
   if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
@@ -892,6 +1054,10 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
   d.hw_p0 = dptr<int32_t>(plan, o_hwp0);
   d.sp_off = sp_off.empty() ? nullptr : dptr<int32_t>(plan, o_spoff);
   d.sp_ent = sp_off.empty() ? nullptr : dptr<int2>(plan, o_spent);
+  d.tc_kind = tc_kind.empty() ? nullptr : dptr<uint32_t>(plan, o_kind);
+  d.opt_kernel = 0;
+  d.opt_lds_list = -1;
+  d.opt_lds_xcd = 1;
   plan->n_sp = (int64_t)sp_ids.size();
   plan->sp_ids = sp_ids.empty() ? nullptr : dptr<int32_t>(plan, o_spids);
   d.piece_row = dptr<int32_t>(plan, o_prow);
@@ -931,7 +1097,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, StilePlan** out) {
 }
 
 int create_spmm(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype, StilePlan** out,
-                bool want_mask = false) {
+                bool want_mask = false, int32_t scatter_max = 0) {
   if (!out) return fail(STILE_E_INVALID, "null output pointer");
   *out = nullptr;
   if (!packs || count <= 0) return fail(STILE_E_INVALID, "no packs");
@@ -961,7 +1127,7 @@ int create_spmm(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype, St
   bd.K = (int32_t)coff;
   bd.inst_rows.push_back(roff);
   bd.inst_cols.push_back(coff);
-  return finish_spmm(bd, blk_dtype, out);
+  return finish_spmm(bd, blk_dtype, scatter_max, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1223,6 +1389,38 @@ int stile_plan_create_spmm(const StileSpmmPack* pack, int32_t blk_dtype, StilePl
 int stile_plan_create_spmm_batched(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype,
                                    StilePlan** out) {
   return create_spmm(packs, count, blk_dtype, out);
+}
+
+int stile_plan_create_spmm_ex(const StileSpmmPack* packs, int32_t count, int32_t blk_dtype,
+                              const StileSpmmOptions* opts, StilePlan** out) {
+  const int32_t scatter = opts ? opts->scatter_max_nnz : 0;
+  if (scatter < 0 || scatter > 16) return fail(STILE_E_INVALID, "scatter_max_nnz must be in 0..16");
+  return create_spmm(packs, count, blk_dtype, out, false, scatter);
+}
+
+int stile_plan_set_option(StilePlan* plan, int32_t option, int32_t value) {
+  if (!plan || plan->kind != 1) return fail(STILE_E_INVALID, "not an SpMM plan");
+  stile::SpmmPlanDev& d = plan->spmm;
+  switch (option) {
+    case STILE_OPT_KERNEL:
+      if (value != 0 && value != 1) return fail(STILE_E_INVALID, "STILE_OPT_KERNEL: 0 or 1");
+      if (value == 1 && d.tc_kind)
+        return fail(STILE_E_UNSUPPORTED, "a plan with scatter chunks runs on the B-stationary kernel only");
+      d.opt_kernel = value;
+      return STILE_OK;
+    case STILE_OPT_LDS_LIST:
+      if (value < -1 || value >= stile::kLdsLists) return fail(STILE_E_INVALID, "STILE_OPT_LDS_LIST: -1 .. 11");
+      if (value >= 0 && d.lds_units && d.lds_cnt[value] == 0)
+        return fail(STILE_E_INVALID, "STILE_OPT_LDS_LIST: the plan has no units in that list");
+      d.opt_lds_list = value;
+      return STILE_OK;
+    case STILE_OPT_LDS_XCD:
+      if (value != 0 && value != 1) return fail(STILE_E_INVALID, "STILE_OPT_LDS_XCD: 0 or 1");
+      d.opt_lds_xcd = value;
+      return STILE_OK;
+    default:
+      return fail(STILE_E_INVALID, "unknown option");
+  }
 }
 
 int stile_plan_create_attention(const StileSpmmPack* packs, int32_t count, StilePlan** out) {

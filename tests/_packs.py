@@ -160,3 +160,30 @@ def sddmm_pack(A, min_fill=64, bucket=32):
                        indptr=np.asarray(indptr), indices_dense=np.concatenate(dense),
                        indices=np.concatenate(outi)))
     return dict(M=M, N=N, nnz=A.nnz, g1d=g1, tc=tc)
+
+
+def add_value_ids(pack, A):
+    """Set a_ids of every group of an SpMM pack built from A: the position of
+    each packed value in A's CSR value order (-1 for padding)."""
+    A = scipy.sparse.csr_matrix(A)
+    A.sort_indices()
+    M, K = A.shape
+    keys = np.repeat(np.arange(M, dtype=np.int64), np.diff(A.indptr)) * K + A.indices
+
+    def ids_of(rows, cols, present):
+        q = np.asarray(rows, np.int64) * K + np.asarray(cols, np.int64)
+        pos = np.searchsorted(keys, np.where(present, q, 0))
+        return np.where(present, pos, -1).astype(np.int32)
+
+    for g in pack["ell"]:
+        Kc = g["nnz_cols"]
+        r = np.repeat(g["indices_reordered_i"], Kc).reshape(-1, Kc)
+        a = g["a"].reshape(-1, Kc)
+        g["a_ids"] = ids_of(r, g["indices_k"], (a != 0) & (r < M)).reshape(g["a"].shape)
+    for g in pack["tc"]:
+        ii = g["i_indices"] if g["rows_2d"] else g["i_indices"][:, None] * 16 + np.arange(16)[None, :]
+        win_of = np.repeat(np.arange(g["num_windows"]), np.diff(g["indptr"]))
+        rows = np.broadcast_to(np.asarray(ii)[win_of][:, :, None], g["a"].shape)
+        cols = np.broadcast_to(g["indices"][:, None, :], g["a"].shape)
+        g["a_ids"] = ids_of(rows, cols, g["a"] != 0)
+    return pack

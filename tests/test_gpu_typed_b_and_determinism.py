@@ -231,27 +231,7 @@ def test_value_refresh_reaches_the_mixed_windows_sparse_rounds(cuda):
     A[70, 5:9] = 1.0
     A = A.tocsr()
     A.sort_indices()
-    pack = _packs.spmm_pack(A, min_fill=64)
-    # value ids: the position of every packed value in A's CSR order
-    M, K = A.shape
-    keys = np.repeat(np.arange(M, dtype=np.int64), np.diff(A.indptr)) * K + A.indices
-
-    def ids_of(rows, cols, present):
-        q = np.asarray(rows, np.int64) * K + np.asarray(cols, np.int64)
-        pos = np.searchsorted(keys, np.where(present, q, 0))
-        return np.where(present, pos, -1).astype(np.int32)
-
-    for g in pack["ell"]:
-        Kc = g["nnz_cols"]
-        r = np.repeat(g["indices_reordered_i"], Kc).reshape(-1, Kc)
-        a = g["a"].reshape(-1, Kc)
-        g["a_ids"] = ids_of(r, g["indices_k"], (a != 0) & (r < M)).reshape(g["a"].shape)
-    for g in pack["tc"]:
-        ii = g["i_indices"] if g["rows_2d"] else g["i_indices"][:, None] * 16 + np.arange(16)[None, :]
-        win_of = np.repeat(np.arange(g["num_windows"]), np.diff(g["indptr"]))
-        rows = np.broadcast_to(np.asarray(ii)[win_of][:, :, None], g["a"].shape)
-        cols = np.broadcast_to(g["indices"][:, None, :], g["a"].shape)
-        g["a_ids"] = ids_of(rows, cols, g["a"] != 0)
+    pack = _packs.add_value_ids(_packs.spmm_pack(A, min_fill=64), A)
     B = rng.standard_normal((512, 96), dtype=np.float32)
     Bd = _dev(B, cuda, torch.bfloat16)
     print("value refresh of a mixed B-stationary plan", flush=True)

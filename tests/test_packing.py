@@ -259,6 +259,23 @@ def test_pack_cache_refuses_another_pattern(tmp_path):
     A2 = A.copy()
     A2.data = A2.data * 3
     assert api.matrix_fingerprint("spmm", A2) == fa
+    # build options travel with the pack; files without them load as {}
+    api.save_pack(tmp_path / "r.npz", pack, fingerprint=fa, build={"scatter_max_nnz": 2})
+    pk, build = api.load_pack(tmp_path / "r.npz", expect=fa, with_build=True)
+    assert build == {"scatter_max_nnz": 2} and pk["M"] == 64
+    assert api.load_pack(tmp_path / "p.npz", expect=fa, with_build=True)[1] == {}
+
+
+def test_plan_variants():
+    """The tuner times scatter chunks off and on only for plans the
+    B-stationary kernel runs (BF16 blocks, every K <= 1024)."""
+    small = datasets.pruned_layer(seed=1, M=64, K=48, N=8)[0]
+    wide = scipy.sparse.random(32, 2000, density=0.01, format="csr", random_state=0)
+    assert set(api.plan_variants("spmm", "bfloat16", small)) == {"", "+scatter2"}
+    assert set(api.plan_variants("spmm", "bfloat16", [small, small])) == {"", "+scatter2"}
+    assert set(api.plan_variants("spmm", "bfloat16", [small, wide])) == {""}
+    assert set(api.plan_variants("spmm", "float32", small)) == {""}
+    assert set(api.plan_variants("sddmm", "bfloat16", small)) == {""}
 
 
 def test_sampled_error_checks_detect_a_wrong_row():
