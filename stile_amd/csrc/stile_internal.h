@@ -110,6 +110,9 @@ struct SpmmPlanDev {
   // k % 8 or k % 8 + 8); nullptr when the plan has none.  A window's scatter
   // chunks follow its dense chunks.
   const uint32_t* tc_kind;
+  // B-stationary plans: first row of window w when its 16 rows are
+  // consecutive, else -1 ([W])
+  const int32_t* win_r0;
   // launch options (stile_plan_set_option): kernel 0 = B-stationary when the
   // plan allows it, 1 = streaming; unit list -1 = the plan's pick, else that
   // list; XCD ranges 1 = on, 0 = off

@@ -1034,13 +1034,39 @@ __device__ __forceinline__ void sparse_round(const TB* __restrict__ B, int64_t l
 // next item's staging keeps the LDS block until every wave is done with it.
 // kMixed: windows may have sparse parts, added (in rounds, p.sp_ent) to the
 // dense sums before the window's rows are stored; a window's first rounds
-// (four, two with FP32 B) are requested when it starts, later ones four at a
-// time when it ends.  Scatter chunks (p.tc_kind) build their A operand from
+// (four, two with FP32 or FP16 B) are requested when it starts, later ones
+// in batches of as many when it ends.  Scatter chunks (p.tc_kind) build their A operand from
 // one k-slot word per lane.
-template <typename TB, bool kMixed>
+// Destination of the B-stationary kernel's stores that fall outside C (rows
+// of -1, columns at or beyond N): every lane stores every time, so that the
+// number of stores a wave has in flight is the same on every path and the
+// waits for its later loads do not wait for them too.  The values written
+// here are never read.
+typedef float f32v4_t __attribute__((ext_vector_type(4)));
+__device__ f32v4_t g_store_sink[64];
+
+// The window's rows as float4 stores of four columns each, one per row
+// (kVec: C rows 16-byte aligned, N a multiple of 4); otherwise store_window.
+template <bool kVec>
+__device__ __forceinline__ void store_window_lds(float* __restrict__ C, int64_t ldc, int N, int col, bool vec,
+                                                 const SlotRows& sr, const f32x4 (&acc)[4]) {
+  if constexpr (kVec) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool ok = sr.row[i] >= 0 && col < N;
+      f32v4_t* dst = ok ? reinterpret_cast<f32v4_t*>(C + (int64_t)sr.row[i] * ldc + col)
+                        : &g_store_sink[threadIdx.x & 63];
+      __builtin_nontemporal_store(f32v4_t{acc[0][i], acc[1][i], acc[2][i], acc[3][i]}, dst);
+    }
+  } else {
+    store_window<true>(C, ldc, N, col, vec, sr, acc);
+  }
+}
+
+template <typename TB, bool kMixed, bool kVec>
 __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __restrict__ tc_ptr,
                                          const int32_t* __restrict__ sp_off, const uint32_t* __restrict__ tc_kind,
-                                         const int4* __restrict__ units,
+                                         const int32_t* __restrict__ win_r0, const int4* __restrict__ units,
                                          const TB* __restrict__ B, int64_t ldb, int N, float* __restrict__ C,
                                          int64_t ldc, int nC, bool vec, bool vec16, int item, lds_u8* lds) {
   constexpr int kWaves = kLdsThreads / 64;
@@ -1076,27 +1102,43 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   const bool has = w_beg < w_end;  // wave-uniform
   const int c_beg = has ? tc_ptr[w_beg] : 0, c_end = has ? tc_ptr[w_end] : 1;
 
-  // The operands of chunk c + 2 are requested while chunk c is multiplied
+  // The operands of chunk c + 3 are requested while chunk c is multiplied
   // (indices clamped to the wave's last chunk, so every load is
-  // unconditional).  Three register sets rotate through a loop unrolled by
-  // three, so no set is copied (a copy would wait for the newest load).
-  // Every window of the run holds at least one chunk, so at most one window
-  // ends before each chunk.  The first window's rows and the first two
-  // chunks' operands are requested before the B block is staged, so their
-  // latency overlaps the staging.
+  // unconditional): a wave's loop is bound by the latency of these requests,
+  // so three are kept in flight.  Four register sets rotate through a loop
+  // unrolled by four, so no set is copied (a copy would wait for the newest
+  // load).  Every window of the run holds at least one chunk, so at most one
+  // window ends before each chunk.  The first window's rows and the first
+  // two chunks' operands are requested before the B block is staged, so
+  // their latency overlaps the staging, the third one after it (fewer
+  // registers live while the staging holds its loads).
   const int last = c_end - 1;
+  // a window's rows: computed from its first row (a scalar load) when they
+  // are consecutive, else loaded
+  auto rows_of = [&](int win) {
+    const int r0 = win_r0[win];
+    SlotRows sr;
+    if (r0 >= 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sr.row[i] = r0 + 4 * g + i;
+    } else {
+      sr = load_win(p, win * 16 + g * 4);
+    }
+    return sr;
+  };
   // chunk kind (scatter or dense): a scalar load of its word of tc_kind
   auto is_sc = [&](int idx) { return tc_kind && ((tc_kind[idx >> 5] >> (idx & 31)) & 1u) != 0; };
   int w = w_beg;
   int w_next = 0;
   SlotRows wr{};
-  TrChunk r0{}, r1{};
+  TrChunk r0{}, r1{}, r2{};
   // kMixed: sp_off of the current window and the next, and the lane's
   // entries of the current window's rounds 0 .. kPfRounds - 1, requested when
   // the window starts and used when it ends.  The loads are unconditional
   // (indices clamped to the last entry; a plan with mixed windows has at
   // least one round), so the wait before their use is an exact count.
-  constexpr int kPfRounds = std::is_same<TB, float>::value ? 2 : 4;  // FP32 B: fewer, for registers
+  constexpr int kPfRounds = std::is_same<TB, __bf16>::value ? 4 : 2;  // FP32 / FP16 B: fewer, for registers
+  constexpr int kBatchRounds = std::is_same<TB, __bf16>::value ? 4 : 2;  // later rounds, per request batch
   int so0 = 0, so1 = 0;
   const int sp_last = kMixed ? sp_off[p.W_tc] - 1 : 0;
   int2 ecur[kPfRounds];
@@ -1106,7 +1148,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   };
   if (has) {
     w_next = tc_ptr[w + 1];
-    wr = load_win(p, w * 16 + g * 4);
+    wr = rows_of(w);
     r0 = load_tr_chunk(p, c_beg, cidx, lane, is_sc(c_beg));
     r1 = load_tr_chunk(p, min(c_beg + 1, last), cidx, lane, is_sc(min(c_beg + 1, last)));
     if constexpr (kMixed) {
@@ -1116,6 +1158,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
     }
   }
   stage_b(lds, B, ldb, koff, ksz, n0, N, vec, vec16);
+  if (has) r2 = load_tr_chunk(p, min(c_beg + 2, last), cidx, lane, is_sc(min(c_beg + 2, last)));
   __syncthreads();
   if (!has) return;
   f32x4 acc[4];
@@ -1128,34 +1171,37 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
 #pragma unroll
         for (int u = 0; u < kPfRounds; ++u)
           if (u < rounds) sparse_round(B, ldb, N, col, vec, lds, koff, ecur[u], acc);
-        // rounds beyond those: entries requested four at a time, one memory
-        // latency per four rounds
-        for (int r = kPfRounds; r < rounds; r += 4) {
-          int2 e[4];
+        // rounds beyond those: entries requested a batch at a time, one
+        // memory latency per batch
+        for (int r = kPfRounds; r < rounds; r += kBatchRounds) {
+          int2 e[kBatchRounds];
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+          for (int u = 0; u < kBatchRounds; ++u)
             e[u] = r + u < rounds ? p.sp_ent[so0 + 16 * (r + u) + eidx] : make_int2(koff, 0);
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+          for (int u = 0; u < kBatchRounds; ++u)
             if (r + u < rounds) sparse_round(B, ldb, N, col, vec, lds, koff, e[u], acc);
         }
       }
     }
-    store_window<true>(C, ldc, N, col, vec, cur, acc);
+    store_window_lds<kVec>(C, ldc, N, col, vec, cur, acc);
   };
-  TrChunk r2;
+  TrChunk r3;
   int c = c_beg;
   auto step = [&](const TrChunk& use, TrChunk& fill) {
-    fill = load_tr_chunk(p, min(c + 2, last), cidx, lane, is_sc(min(c + 2, last)));
+    fill = load_tr_chunk(p, min(c + 3, last), cidx, lane, is_sc(min(c + 3, last)));
     if (c >= w_next) {  // window w is complete: store it
-      const SlotRows wn = load_win(p, (w + 1) * 16 + g * 4);
+      // The next window's rows are loaded into wr before the stores: a copy
+      // of a load result after the (conditional) stores would make the wave
+      // wait for every store it just issued.
+      const SlotRows cur = wr;
+      wr = rows_of(w + 1);
       const int w_next2 = tc_ptr[w + 2];
       int so2 = 0;
       if constexpr (kMixed) so2 = sp_off[w + 2];
-      finish(wr);
+      finish(cur);
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-      wr = wn;
       ++w;
       w_next = w_next2;
       if constexpr (kMixed) {
@@ -1167,13 +1213,15 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
     tr_chunk_mfma(lds, cst, use, acc);
     ++c;
   };
-  while (c + 3 <= c_end) {
-    step(r0, r2);
+  while (c + 4 <= c_end) {
+    step(r0, r3);
     step(r1, r0);
     step(r2, r1);
+    step(r3, r2);
   }
-  if (c < c_end) step(r0, r2);
+  if (c < c_end) step(r0, r3);
   if (c < c_end) step(r1, r0);
+  if (c < c_end) step(r2, r1);
   finish(wr);
 }
 
@@ -1192,11 +1240,12 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
 // 16i + 15 of tail_tasks.  Under the XCD grouping those are the last slot of
 // each XCD, which holds the fewest items when a range does not divide evenly;
 // one-item launches add tail blocks past the items when CUs are free.
-template <bool kLoop, typename TB, bool kMixed>
+template <bool kLoop, typename TB, bool kMixed, bool kVec>
 __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlanDev p,
                                                                         const int32_t* __restrict__ tc_ptr,
                                                                         const int32_t* __restrict__ sp_off,
                                                                         const uint32_t* __restrict__ tc_kind,
+                                                                        const int32_t* __restrict__ win_r0,
                                                                         const int4* __restrict__ units,
                                                                         const TB* __restrict__ B,
                                                                         int64_t ldb, int N, float* __restrict__ C,
@@ -1208,7 +1257,8 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   const int tb = (int)gridDim.x - 1 - (int)blockIdx.x;
   if (!kLoop) {
     if ((int)blockIdx.x < n_items)
-      lds_item<TB, kMixed>(p, tc_ptr, sp_off, tc_kind, units, B, ldb, N, C, ldc, nC, vec, vec16, blockIdx.x, lds);
+      lds_item<TB, kMixed, kVec>(p, tc_ptr, sp_off, tc_kind, win_r0, units, B, ldb, N, C, ldc, nC, vec, vec16,
+                                 blockIdx.x, lds);
     if (tb < tail_blocks) {
       const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
       if (task < tail_tasks) sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, task);
@@ -1231,7 +1281,8 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   }
   for (int item = beg; item < end; item += step) {
     __syncthreads();  // every wave is done reading the previous item's block
-    lds_item<TB, kMixed>(p, tc_ptr, sp_off, tc_kind, units, B, ldb, N, C, ldc, nC, vec, vec16, item, lds);
+    lds_item<TB, kMixed, kVec>(p, tc_ptr, sp_off, tc_kind, win_r0, units, B, ldb, N, C, ldc, nC, vec, vec16, item,
+                               lds);
   }
   if (tb < tail_blocks) {
     const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
@@ -1323,8 +1374,10 @@ __global__ __launch_bounds__(256) void hyb_spmm_piece_reduce_kernel(SpmmPlanDev 
 
 template <bool kLoop, typename TB>
 static void set_lds_bytes(int bytes) {
-  for (const void* f : {reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<kLoop, TB, false>),
-                        reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<kLoop, TB, true>)})
+  for (const void* f : {reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<kLoop, TB, false, false>),
+                        reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<kLoop, TB, true, false>),
+                        reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<kLoop, TB, false, true>),
+                        reinterpret_cast<const void*>(hyb_spmm_lds_bf16_kernel<kLoop, TB, true, true>)})
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
@@ -1418,17 +1471,25 @@ static hipError_t launch_hyb_spmm_t(const SpmmPlanDev& p, const TB* B, int64_t l
     const size_t lds_bytes = (size_t)p.lds_max_k * kLdsRowBytes;
     const int4* units = p.lds_units + p.lds_off[t];
     const bool mixed = p.n_mix > 0 && p.sp_off != nullptr;
-#define STILE_LDS_LAUNCH(LOOP, MIX)                                                                          \
-  hipLaunchKernelGGL((hyb_spmm_lds_bf16_kernel<LOOP, TB, MIX>), dim3(grid), dim3(kLdsThreads), lds_bytes, s, p, \
-                     p.tc_ptr, p.sp_off, p.tc_kind, units, B, ldb, N, C, ldc, nC, (int)items, vec, vec16, xr,      \
-                     LOOP && xcd,                                                                              \
-                     tail_blocks, tail_tasks)
+#define STILE_LDS_LAUNCH1(LOOP, MIX, VEC)                                                                    \
+  hipLaunchKernelGGL((hyb_spmm_lds_bf16_kernel<LOOP, TB, MIX, VEC>), dim3(grid), dim3(kLdsThreads), lds_bytes, s, \
+                     p,                                                                                        \
+                     p.tc_ptr, p.sp_off, p.tc_kind, p.win_r0, units, B, ldb, N, C, ldc, nC, (int)items, vec, vec16, \
+                     xr, LOOP && xcd, tail_blocks, tail_tasks)
+#define STILE_LDS_LAUNCH(LOOP, MIX)                \
+  do {                                             \
+    if (vec)                                       \
+      STILE_LDS_LAUNCH1(LOOP, MIX, true);          \
+    else                                           \
+      STILE_LDS_LAUNCH1(LOOP, MIX, false);         \
+  } while (0)
     if (items <= cus) {
       if (mixed) STILE_LDS_LAUNCH(false, true); else STILE_LDS_LAUNCH(false, false);
     } else {
       if (mixed) STILE_LDS_LAUNCH(true, true); else STILE_LDS_LAUNCH(true, false);
     }
 #undef STILE_LDS_LAUNCH
+#undef STILE_LDS_LAUNCH1
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   } else if (p.W_tc > 0) {

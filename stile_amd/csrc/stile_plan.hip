@@ -980,6 +980,16 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, int32_t scatter_max, StilePlan
   const size_t o_spent = lay.add(sp_ent.size() * 4);
   const size_t o_spids = lay.add(sp_ids.size() * 4);
   const size_t o_kind = lay.add(tc_kind.size() * 4);
+  // first row of each window whose 16 rows are consecutive (else -1): the
+  // B-stationary kernel then computes the rows instead of loading them
+  std::vector<int32_t> win_r0(lds_ok ? W : 0, -1);
+  for (int32_t w = 0; lds_ok && w < W; ++w) {
+    const int32_t r0 = bd.win_rows[(size_t)w * 16];
+    bool run = r0 >= 0;
+    for (int r = 1; r < 16 && run; ++r) run = bd.win_rows[(size_t)w * 16 + r] == r0 + r;
+    if (run) win_r0[w] = r0;
+  }
+  const size_t o_r0 = lay.add(win_r0.size() * 4);
   HostBlob blob;
   blob.data.assign(lay.bytes, 0);
   blob.put(o_rows, bd.win_rows.data(), bd.win_rows.size() * 4);
@@ -1009,6 +1019,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, int32_t scatter_max, StilePlan
   blob.put(o_spent, sp_ent.data(), sp_ent.size() * 4);
   blob.put(o_spids, sp_ids.data(), sp_ids.size() * 4);
   blob.put(o_kind, tc_kind.data(), tc_kind.size() * 4);
+  blob.put(o_r0, win_r0.data(), win_r0.size() * 4);
 
   StilePlan* plan = new (std::nothrow) StilePlan();
   if (!plan) return fail(STILE_E_NOMEM, "host allocation of plan");
@@ -1055,6 +1066,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, int32_t scatter_max, StilePlan
   d.sp_off = sp_off.empty() ? nullptr : dptr<int32_t>(plan, o_spoff);
   d.sp_ent = sp_off.empty() ? nullptr : dptr<int2>(plan, o_spent);
   d.tc_kind = tc_kind.empty() ? nullptr : dptr<uint32_t>(plan, o_kind);
+  d.win_r0 = win_r0.empty() ? nullptr : dptr<int32_t>(plan, o_r0);
   d.opt_kernel = 0;
   d.opt_lds_list = -1;
   d.opt_lds_xcd = 1;
