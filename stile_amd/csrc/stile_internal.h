@@ -88,6 +88,11 @@ struct SpmmPlanDev {
   const int32_t* piece_row;  // [npieces] output row
   const int32_t* piece_beg;  // [npieces] range in res_cols/res_vals
   const int32_t* piece_end;
+  // execution order of the pieces for the fix-up kernel: within [0,
+  // npieces_dense) and [npieces_dense, npieces), pieces by their first
+  // column, so the pieces running together (of different long rows) gather
+  // the same B rows ([npieces])
+  const int32_t* piece_exec;
   // split rows (deterministic mode): row r's pieces are [srow_p0[r], srow_p1[r])
   int32_t nsrows;
   int32_t nsrows_dense;      // split rows [0, nsrows_dense) are slots of dense windows

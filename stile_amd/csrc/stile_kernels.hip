@@ -1290,7 +1290,9 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   }
 }
 
-// Remaining pieces of long sparse rows, pieces [pc0, pc1): lane group g of
+// Remaining pieces of long sparse rows, execution slots [pc0, pc1) (piece
+// p.piece_exec[slot], ordered by first column so that pieces of different
+// rows gathering the same B rows run together): lane group g of
 // the wave takes nonzeros q = beg + g, beg + g + 4, ...; the four partial
 // float4 are reduced across groups and added to C with atomicAdd (C already
 // holds the window pass's value of the row), or, with ws (deterministic
@@ -1301,9 +1303,10 @@ __global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, cons
                                                              int64_t ldc, int nC, bool vec, int pc0, int pc1,
                                                              float* __restrict__ ws) {
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int pc = pc0 + wave / nC;
-  if (pc >= pc1) return;
-  const int n0 = (wave - (pc - pc0) * nC) * 64;
+  const int pe = pc0 + wave / nC;  // execution slot; the piece is p.piece_exec[pe]
+  if (pe >= pc1) return;
+  const int pc = p.piece_exec[pe];
+  const int n0 = (wave - (pe - pc0) * nC) * 64;
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
   const int col = n0 + 4 * (lane & 15);

@@ -961,7 +961,17 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, int32_t scatter_max, StilePlan
   const size_t o_resptr = lay.add(res_ptr.size() * 4);
   const size_t o_rescols = lay.add(res_cols.size() * 4);
   const size_t o_resvals = lay.add(res_vals.size() * 4);
+  // the fix-up kernel's piece order (see SpmmPlanDev::piece_exec)
+  std::vector<int32_t> piece_exec(piece_row.size());
+  for (size_t q = 0; q < piece_exec.size(); ++q) piece_exec[q] = (int32_t)q;
+  auto by_first_col = [&](int32_t a, int32_t b) {
+    const int32_t ca = res_cols[piece_beg[a]], cb = res_cols[piece_beg[b]];
+    return ca != cb ? ca < cb : a < b;
+  };
+  std::sort(piece_exec.begin(), piece_exec.begin() + npieces_dense, by_first_col);
+  std::sort(piece_exec.begin() + npieces_dense, piece_exec.end(), by_first_col);
   const size_t o_prow = lay.add(piece_row.size() * 4);
+  const size_t o_pexec = lay.add(piece_exec.size() * 4);
   const size_t o_pbeg = lay.add(piece_beg.size() * 4);
   const size_t o_pend = lay.add(piece_end.size() * 4);
   const size_t o_tcids = lay.add(tc_ids.size() * 4);
@@ -1001,6 +1011,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, int32_t scatter_max, StilePlan
   blob.put(o_rescols, res_cols.data(), res_cols.size() * 4);
   blob.put(o_resvals, res_vals.data(), res_vals.size() * 4);
   blob.put(o_prow, piece_row.data(), piece_row.size() * 4);
+  blob.put(o_pexec, piece_exec.data(), piece_exec.size() * 4);
   blob.put(o_pbeg, piece_beg.data(), piece_beg.size() * 4);
   blob.put(o_pend, piece_end.data(), piece_end.size() * 4);
   blob.put(o_tcids, tc_ids.data(), tc_ids.size() * 4);
@@ -1073,6 +1084,7 @@ int finish_spmm(SpmmBuild& bd, int32_t blk_dtype, int32_t scatter_max, StilePlan
   plan->n_sp = (int64_t)sp_ids.size();
   plan->sp_ids = sp_ids.empty() ? nullptr : dptr<int32_t>(plan, o_spids);
   d.piece_row = dptr<int32_t>(plan, o_prow);
+  d.piece_exec = dptr<int32_t>(plan, o_pexec);
   d.piece_beg = dptr<int32_t>(plan, o_pbeg);
   d.piece_end = dptr<int32_t>(plan, o_pend);
   d.blk_dtype = blk_dtype;
