@@ -532,7 +532,9 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
     each, compared by median.  kind: 'spmm' (width = N, B of type b_dtype: float32, bfloat16
     or float16, the type the plan will be run with) or 'sddmm' (width =
     K_feat, FP32 blocks).  Every searched pack is timed under each build
-    variant (plan_variants: scatter chunks off / on), named search + variant.
+    variant (plan_variants: scatter chunks off / on), named search + variant;
+    for a B-stationary choice every work-unit list is then timed too and the
+    fastest kept as a launch option (build["launch"]).
     report: {"choice": name, "build": build options of the choice, "ms":
     {name: ms per run}, "search_s": {search: seconds}, "err": {name: sampled
     relative error}}.  cache: an .npz path; if it exists and was saved for
@@ -546,8 +548,13 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
     if kind == "sddmm":
         blk_dtype = "float32"
     fp = matrix_fingerprint(kind, mats)
-    make = (lambda pk, bo: _native.Plan.spmm(pk, blk_dtype, **bo)) if kind == "spmm" else \
-        (lambda pk, bo: _native.Plan.sddmm(pk, "float32"))
+    def make(pk, bo):
+        if kind != "spmm":
+            return _native.Plan.sddmm(pk, "float32")
+        plan = _native.Plan.spmm(pk, blk_dtype, **{k: v for k, v in bo.items() if k != "launch"})
+        for name, value in (bo.get("launch") or {}).items():
+            plan.set_option(name, value)
+        return plan
     M, K = int(_offsets(mats, 0)[-1]), int(_offsets(mats, 1)[-1])
     dev = torch.device("cuda")
     g = torch.Generator(device=dev)
@@ -634,10 +641,52 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
     if best is None:
         raise RuntimeError(f"tune {kind}: no candidate plan passed the float64 check ({errs})")
     sname, build = origin[choice]
+    build = dict(build)
+    lists = _time_unit_lists(best, runs[choice], reps, rounds) if len(variants) > 1 else {}
+    if lists:
+        t_best = min(lists, key=lists.get)
+        best.set_option("lds_list", t_best)
+        if t_best >= 0:
+            build["launch"] = {"lds_list": t_best}
+        if log:
+            log(f"tune {kind} {choice}: work-unit list {t_best} ({lists[t_best] * 1e3:.2f} us; "
+                f"the plan's pick {lists.get(-1, float('nan')) * 1e3:.2f} us)")
     if cache is not None:
         os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
         save_pack(cache, packs[sname], fingerprint=fp, build=build)
-    return best, packs[sname], {"choice": choice, "build": build, "ms": times, "search_s": search_s, "err": errs}
+    return best, packs[sname], {"choice": choice, "build": build, "ms": times, "search_s": search_s, "err": errs,
+                                "lds_list_ms": {str(k): round(v, 5) for k, v in lists.items()}}
+
+
+def _time_unit_lists(plan, run, reps, rounds):
+    """Median ms per run of a B-stationary plan under each work-unit list it
+    accepts (-1: the plan's own pick), timed in turn over `rounds` rounds.
+    The launch option is left at -1."""
+    import torch
+    from ._native import StileError
+    lists = []
+    for t in [-1] + list(range(12)):
+        try:
+            plan.set_option("lds_list", t)
+        except StileError:
+            continue
+        lists.append(t)
+    samples = {t: [] for t in lists}
+    for _ in range(rounds):
+        for t in lists:
+            plan.set_option("lds_list", t)
+            for _ in range(3):
+                run()
+            torch.cuda.synchronize()
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record()
+            for _ in range(reps):
+                run()
+            e.record()
+            torch.cuda.synchronize()
+            samples[t].append(s.elapsed_time(e) / reps)
+    plan.set_option("lds_list", -1)
+    return {t: float(np.median(v)) for t, v in samples.items()}
 
 
 def pack_nnz(pack):

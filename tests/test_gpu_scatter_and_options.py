@@ -165,3 +165,26 @@ def test_launch_options(cuda):
         sc.set_option("kernel", 1)
     plan.close()
     sc.close()
+
+
+def test_tuner_times_build_and_launch_variants(cuda, tmp_path):
+    """tune_plan on a B-stationary matrix times every search setting with and
+    without scatter chunks, then the work-unit lists of its choice; the plan
+    rebuilt from the cache (pack + build + launch options) gives the same
+    output bit for bit, and both are within the BF16 bar against FP64."""
+    import torch
+    A, _ = datasets.pruned_layer(seed=4, M=1024, K=512, N=1)
+    cache = str(tmp_path / "tuned.npz")
+    print("tuner variants", flush=True)
+    plan, pack, rep = api.tune_plan("spmm", A, "bfloat16", 128, b_dtype="bfloat16", cache=cache, reps=5, rounds=3)
+    assert any(k.endswith("+scatter2") for k in rep["ms"]) and rep["lds_list_ms"]
+    assert "-1" in rep["lds_list_ms"]
+    B = np.random.default_rng(12).standard_normal((512, 128), dtype=np.float32)
+    Bd = _dev(B, cuda, torch.bfloat16)
+    C1 = plan.run_spmm(Bd).cpu().numpy()
+    assert ops.max_rel_err(C1, ops.spmm_fp64(A, _exact(Bd))) <= 1e-2
+    plan2, _, rep2 = api.tune_plan("spmm", A, "bfloat16", 128, b_dtype="bfloat16", cache=cache)
+    assert rep2["choice"] == "cached" and rep2["build"] == rep["build"]
+    assert np.array_equal(plan2.run_spmm(Bd).cpu().numpy(), C1)
+    plan.close()
+    plan2.close()
