@@ -432,6 +432,51 @@ def plan_variants(kind, blk_dtype, mats):
     return {"": {}}
 
 
+def spmm_plan_signature(packs):
+    """SHA-1 of what a device plan built from SpMM packs (one or a list)
+    depends on: per matrix the (row, column) of every nonzero held by TC
+    tiles, the row sets of the TC windows, and the (row, column) of every
+    nonzero held by ELL tiles.  Packs that differ only in how the TC
+    nonzeros are grouped into tiles have the same signature and give the same
+    plan: the plan builder merges a window's tiles into one column list."""
+    import hashlib
+    h = hashlib.sha1()
+    for pk in _as_list(packs):
+        M, K = int(pk["M"]), int(pk["K"])
+        tc, wins, ell = [], [], []
+        for g in pk["ell"]:
+            I, Kc, T = int(g["m_blk"]), int(g["nnz_cols"]), int(g["num_tiles"])
+            a = np.asarray(g["a"]).reshape(T * I, Kc)
+            r = np.repeat(np.asarray(g["indices_reordered_i"], np.int64), Kc).reshape(T * I, Kc)
+            m = (a != 0) & (r < M)
+            if g.get("a_ids") is not None:
+                m |= (np.asarray(g["a_ids"]).reshape(T * I, Kc) >= 0) & (r < M)
+            ell.append(r[m] * K + np.asarray(g["indices_k"], np.int64).reshape(T * I, Kc)[m])
+        for g in pk["tc"]:
+            T, Kg, W = int(g["nnzb"]), int(g["group_size"]), int(g["num_windows"])
+            ii = np.asarray(g["i_indices"], np.int64)
+            rr = ii.reshape(W, 16) if g["rows_2d"] else ii[:, None] * 16 + np.arange(16)[None, :]
+            rr = np.where(rr < M, rr, -1)
+            wins.append(rr)
+            win_of = np.repeat(np.arange(W), np.diff(np.asarray(g["indptr"], np.int64)))
+            a = np.asarray(g["a"]).reshape(T, 16, Kg)
+            present = a != 0
+            if g.get("a_ids") is not None:
+                present |= np.asarray(g["a_ids"]).reshape(T, 16, Kg) >= 0
+            if g["k_notsorted"]:
+                cols = np.asarray(g["indices"], np.int64)[:, None] * 16 + np.arange(16)[None, :]
+            else:
+                cols = np.asarray(g["indices"], np.int64).reshape(T, Kg)
+            t, r, k = np.nonzero(present)
+            tc.append(rr[win_of[t], r] * K + cols[t, k])
+        for arr in (np.unique(np.concatenate(tc)) if tc else np.zeros(0, np.int64),
+                    np.unique(np.concatenate(wins), axis=0) if wins else np.zeros((0, 16), np.int64),
+                    np.unique(np.concatenate(ell)) if ell else np.zeros(0, np.int64)):
+            h.update(np.ascontiguousarray(arr, np.int64).tobytes())
+            h.update(b"|")
+    return h.hexdigest()
+
+
 def _pack_any(kind, mats, opts, width, workers, log=None):
     if isinstance(mats, (list, tuple)):
         job = [(kind, A, opts, width) for A in mats]
@@ -591,10 +636,19 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
         if log:
             log(f"tune {kind}: searched {name} in {search_s[name]} s")
     variants = plan_variants(kind, blk_dtype, mats)
+    # searches that produced the same plan (the same nonzeros in TC tiles,
+    # windows and ELL tiles) are built and timed once, under the first name
+    same_as = {}
+    if kind == "spmm":
+        first = {}
+        for sname, pack in packs.items():
+            same_as[sname] = first.setdefault(spmm_plan_signature(pack), sname)
     times, errs, plans, runs, origin = {}, {}, {}, {}, {}
     for (sname, pack), (vname, build) in ((sp, v) for sp in packs.items() for v in variants.items()):
         name = sname + vname
         origin[name] = (sname, build)
+        if same_as.get(sname, sname) != sname:
+            continue
         plan = make(pack, build)
         err = check(plan, pack)
         errs[name] = float(f"{err:.4g}")
@@ -614,9 +668,13 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
     # their median: a drift of the clocks during tuning then shifts every
     # candidate alike instead of favouring the one timed first
     samples = {name: [] for name in plans}
-    for _ in range(rounds):
-        for name, run in runs.items():
-            for _ in range(3):
+    names = list(runs)
+    for rd in range(rounds):
+        # each round starts at another candidate, so none always follows the
+        # same (possibly very different) plan
+        for name in names[rd % len(names):] + names[:rd % len(names)]:
+            run = runs[name]
+            for _ in range(10):
                 run()
             torch.cuda.synchronize()
             s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -638,6 +696,11 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
     for name, plan in plans.items():
         if name != choice:
             plan.close()
+    for name, (sname, build) in origin.items():       # duplicates report their twin's figures
+        twin = same_as.get(sname, sname) + name[len(sname):]
+        if twin != name and twin in times:
+            times[name] = times[twin]
+            errs[name] = errs.get(twin)
     if best is None:
         raise RuntimeError(f"tune {kind}: no candidate plan passed the float64 check ({errs})")
     sname, build = origin[choice]
@@ -655,6 +718,7 @@ def tune_plan(kind, mats, blk_dtype="bfloat16", width=128, candidates=None, reps
         os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
         save_pack(cache, packs[sname], fingerprint=fp, build=build)
     return best, packs[sname], {"choice": choice, "build": build, "ms": times, "search_s": search_s, "err": errs,
+                                "same_plan_as": {k: v for k, v in same_as.items() if k != v},
                                 "lds_list_ms": {str(k): round(v, 5) for k, v in lists.items()}}
 
 
@@ -672,10 +736,10 @@ def _time_unit_lists(plan, run, reps, rounds):
             continue
         lists.append(t)
     samples = {t: [] for t in lists}
-    for _ in range(rounds):
-        for t in lists:
+    for rd in range(rounds):
+        for t in lists[rd % len(lists):] + lists[:rd % len(lists)]:
             plan.set_option("lds_list", t)
-            for _ in range(3):
+            for _ in range(10):
                 run()
             torch.cuda.synchronize()
             s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -295,3 +295,21 @@ def test_sampled_error_checks_detect_a_wrong_row():
     assert api.sddmm_sample_error(mats, X, Y, out, samples=big.nnz) < 1e-6
     out[big.nnz - 1] += 1.0
     assert api.sddmm_sample_error(mats, X, Y, out, samples=big.nnz) > 1e-2
+
+
+def test_spmm_plan_signature():
+    """Packs that group the same TC nonzeros into tiles differently give one
+    plan, hence one signature; moving nonzeros between TC and ELL tiles, or
+    another matrix, changes it."""
+    from tests import _packs
+    A = datasets.pruned_layer(seed=2, M=128, K=96, N=8)[0]
+    p16 = _packs.spmm_pack(A, min_fill=64, tc_group_cols=16)
+    p32 = _packs.spmm_pack(A, min_fill=64, tc_group_cols=32)
+    assert [g["group_size"] for g in p16["tc"]] != [g["group_size"] for g in p32["tc"]]
+    assert api.spmm_plan_signature(p16) == api.spmm_plan_signature(p32)
+    p_more_ell = _packs.spmm_pack(A, min_fill=257)       # every nonzero in ELL tiles
+    assert api.spmm_plan_signature(p_more_ell) != api.spmm_plan_signature(p16)
+    B = datasets.pruned_layer(seed=3, M=128, K=96, N=8)[0]
+    assert api.spmm_plan_signature(_packs.spmm_pack(B, min_fill=64)) != api.spmm_plan_signature(p16)
+    assert api.spmm_plan_signature([p16, p16]) != api.spmm_plan_signature([p16])
+    assert api.spmm_plan_signature([p16]) == api.spmm_plan_signature(p16)
