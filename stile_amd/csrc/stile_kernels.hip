@@ -1037,26 +1037,28 @@ __device__ __forceinline__ void sparse_round(const TB* __restrict__ B, int64_t l
 // (four, two with FP32 or FP16 B) are requested when it starts, later ones
 // in batches of as many when it ends.  Scatter chunks (p.tc_kind) build their A operand from
 // one k-slot word per lane.
-// Destination of the B-stationary kernel's stores that fall outside C (rows
-// of -1, columns at or beyond N): every lane stores every time, so that the
-// number of stores a wave has in flight is the same on every path and the
-// waits for its later loads do not wait for them too.  The values written
-// here are never read.
 typedef float f32v4_t __attribute__((ext_vector_type(4)));
-__device__ f32v4_t g_store_sink[64];
 
-// The window's rows as float4 stores of four columns each, one per row
-// (kVec: C rows 16-byte aligned, N a multiple of 4); otherwise store_window.
+// The window's rows as float4 stores of four columns each, one per row.
+// kVec (C rows 16-byte aligned, N a multiple of 4, C below 2 GiB): buffer
+// stores with L2 write-through (sc1) over a descriptor of [C, C + cbytes),
+// measured 4 % faster on the block-pruned layer at N = 4096 than
+// non-temporal global stores; stores that fall outside C (rows of -1,
+// columns at or beyond N) get an offset past the descriptor, which the
+// hardware drops, so every lane issues the same stores on every path.
+// Otherwise store_window.
 template <bool kVec>
 __device__ __forceinline__ void store_window_lds(float* __restrict__ C, int64_t ldc, int N, int col, bool vec,
-                                                 const SlotRows& sr, const f32x4 (&acc)[4]) {
+                                                 int cbytes, const SlotRows& sr, const f32x4 (&acc)[4]) {
   if constexpr (kVec) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(C, 0, cbytes, 0x00020000);
+    constexpr uint32_t kOut = 0x80000000u;  // past every descriptor (cbytes < 2^31)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const bool ok = sr.row[i] >= 0 && col < N;
-      f32v4_t* dst = ok ? reinterpret_cast<f32v4_t*>(C + (int64_t)sr.row[i] * ldc + col)
-                        : &g_store_sink[threadIdx.x & 63];
-      __builtin_nontemporal_store(f32v4_t{acc[0][i], acc[1][i], acc[2][i], acc[3][i]}, dst);
+      const uint32_t off = ok ? ((uint32_t)sr.row[i] * (uint32_t)ldc + (uint32_t)col) * 4u : kOut;
+      __builtin_amdgcn_raw_buffer_store_b128(f32v4_t{acc[0][i], acc[1][i], acc[2][i], acc[3][i]}, rs, (int)off, 0,
+                                            16);
     }
   } else {
     store_window<true>(C, ldc, N, col, vec, sr, acc);
@@ -1068,7 +1070,8 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
                                          const int32_t* __restrict__ sp_off, const uint32_t* __restrict__ tc_kind,
                                          const int32_t* __restrict__ win_r0, const int4* __restrict__ units,
                                          const TB* __restrict__ B, int64_t ldb, int N, float* __restrict__ C,
-                                         int64_t ldc, int nC, bool vec, bool vec16, int item, lds_u8* lds) {
+                                         int64_t ldc, int nC, bool vec, bool vec16, int cbytes, int item,
+                                         lds_u8* lds) {
   constexpr int kWaves = kLdsThreads / 64;
   const int chunk = item % nC;
   const int4 unit = units[item / nC];
@@ -1184,7 +1187,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
         }
       }
     }
-    store_window_lds<kVec>(C, ldc, N, col, vec, cur, acc);
+    store_window_lds<kVec>(C, ldc, N, col, vec, cbytes, cur, acc);
   };
   TrChunk r3;
   int c = c_beg;
@@ -1250,15 +1253,15 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
                                                                         const TB* __restrict__ B,
                                                                         int64_t ldb, int N, float* __restrict__ C,
                                                                         int64_t ldc, int nC, int n_items, bool vec,
-                                                                        bool vec16, LdsXcdRanges xr, bool xcd,
-                                                                        int tail_blocks, int tail_tasks) {
+                                                                        bool vec16, int cbytes, LdsXcdRanges xr,
+                                                                        bool xcd, int tail_blocks, int tail_tasks) {
   extern __shared__ uint4 lds_dyn[];  // [ksz][kLdsRowBytes]
   lds_u8* lds = (lds_u8*)lds_dyn;
   const int tb = (int)gridDim.x - 1 - (int)blockIdx.x;
   if (!kLoop) {
     if ((int)blockIdx.x < n_items)
       lds_item<TB, kMixed, kVec>(p, tc_ptr, sp_off, tc_kind, win_r0, units, B, ldb, N, C, ldc, nC, vec, vec16,
-                                 blockIdx.x, lds);
+                                 cbytes, blockIdx.x, lds);
     if (tb < tail_blocks) {
       const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
       if (task < tail_tasks) sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, task);
@@ -1281,8 +1284,8 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   }
   for (int item = beg; item < end; item += step) {
     __syncthreads();  // every wave is done reading the previous item's block
-    lds_item<TB, kMixed, kVec>(p, tc_ptr, sp_off, tc_kind, win_r0, units, B, ldb, N, C, ldc, nC, vec, vec16, item,
-                               lds);
+    lds_item<TB, kMixed, kVec>(p, tc_ptr, sp_off, tc_kind, win_r0, units, B, ldb, N, C, ldc, nC, vec, vec16, cbytes,
+                               item, lds);
   }
   if (tb < tail_blocks) {
     const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
@@ -1474,14 +1477,18 @@ static hipError_t launch_hyb_spmm_t(const SpmmPlanDev& p, const TB* B, int64_t l
     const size_t lds_bytes = (size_t)p.lds_max_k * kLdsRowBytes;
     const int4* units = p.lds_units + p.lds_off[t];
     const bool mixed = p.n_mix > 0 && p.sp_off != nullptr;
+    // C addressed by a buffer descriptor (the kVec variants) when its extent
+    // is below 2 GiB
+    const int64_t c_extent = ((int64_t)(p.M - 1) * ldc + N) * 4;
+    const int cbytes = p.M > 0 && c_extent < ((int64_t)1 << 31) ? (int)c_extent : 0;
 #define STILE_LDS_LAUNCH1(LOOP, MIX, VEC)                                                                    \
   hipLaunchKernelGGL((hyb_spmm_lds_bf16_kernel<LOOP, TB, MIX, VEC>), dim3(grid), dim3(kLdsThreads), lds_bytes, s, \
                      p,                                                                                        \
                      p.tc_ptr, p.sp_off, p.tc_kind, p.win_r0, units, B, ldb, N, C, ldc, nC, (int)items, vec, vec16, \
-                     xr, LOOP && xcd, tail_blocks, tail_tasks)
+                     cbytes, xr, LOOP && xcd, tail_blocks, tail_tasks)
 #define STILE_LDS_LAUNCH(LOOP, MIX)                \
   do {                                             \
-    if (vec)                                       \
+    if (vec && cbytes > 0)                         \
       STILE_LDS_LAUNCH1(LOOP, MIX, true);          \
     else                                           \
       STILE_LDS_LAUNCH1(LOOP, MIX, false);         \
