@@ -3,10 +3,14 @@
 Default workload (one step = one launch of the fused SpMM):
   C = A . B, A the 3072 x 768 block-pruned layer (10 % of the 16x16 blocks
   dense + 1 % unstructured residual, 257,265 nonzeros, seed 0), B 768 x 4096
-  FP32 resident in HBM, C 3072 x 4096 FP32.  Tiles from the STile search
+  BF16 resident in HBM, C 3072 x 4096 FP32.  Tiles from the STile search
   (greedy + withdraw, cost model), the search setting picked by measured
   latency (api.tune_plan); dense blocks run BF16 MFMA with FP32
-  accumulation, the sparse rest FP32.
+  accumulation, the sparse rest FP32.  Sub-objects of the same line: the
+  layer with FP32 B (b_fp32) and at N = 128 (n128), the encoder batch below
+  at N = 128 (pruned_batch_n128), and the uniform SpMM and R-MAT SDDMM below
+  (uniform_spmm_n64, rmat_sddmm_k32; --no-n128, --no-batch, --no-small skip
+  them).
 
 Other workloads (--workload), each printed as its own JSON line:
   pruned_batch  120 block-pruned matrices of a 24-layer BERT-large-shaped
@@ -328,6 +332,13 @@ def run_pruned_layer(args, dev):
         out["cpu_torch_csr_all_threads"] = cpu_torch_csr_baseline(A, B, 512, 5.0)
     if not args.no_batch:
         out["pruned_batch_n128"] = pruned_batch_summary(args, dev)
+    if not args.no_small:
+        # the two small configurations, so that the default run times them too
+        keep = ("metric", "ms_per_step", "value", "unit", "roofline", "hybrid_over_csr", "hybrid_over_vendor",
+                "max_rel_err_vs_fp64", "config")
+        for name, fn in (("uniform_spmm_n64", run_uniform_spmm), ("rmat_sddmm_k32", run_rmat_sddmm)):
+            r = fn(args, dev, summary=True)
+            out[name] = {k: r[k] for k in keep if k in r}
     return out
 
 
@@ -400,7 +411,7 @@ def run_pruned_batch(args, dev, summary=False):
     return out
 
 
-def run_uniform_spmm(args, dev):
+def run_uniform_spmm(args, dev, summary=False):
     import torch
     from oracle import ops
     from stile_amd import api, datasets
@@ -425,13 +436,14 @@ def run_uniform_spmm(args, dev):
                hybrid_over_vendor=None if ms_vendor is None else round(ms_vendor / ms, 3),
                hybrid_over_sparsetir_hyb=round(ms_hyb / ms, 3),
                plan=dict(plan.info(), search_tune_and_plan_s=round(t_plan, 3), tuning=tuning))
-    if not args.no_cpu:
+    if summary or not args.no_cpu:
         out["max_rel_err_vs_fp64"] = ops.max_rel_err(C.cpu().numpy(), ops.spmm_fp64(A, B))
+    if not summary and not args.no_cpu:
         out["cpu_baseline"] = cpu_spmm_baseline(A, B, N, 10.0, "4096x4096")
     return out
 
 
-def run_rmat_sddmm(args, dev):
+def run_rmat_sddmm(args, dev, summary=False):
     import numpy as np
     import torch
     from oracle import ops
@@ -455,10 +467,11 @@ def run_rmat_sddmm(args, dev):
                roofline=roofline(sddmm_alg_bytes(A.nnz, M, Nc, Kf), ms),
                plain_csr_gflops=_gf(flops, ms_csr), hybrid_over_csr=round(ms_csr / ms, 3),
                plan=dict(plan.info(), search_tune_and_plan_s=round(t_plan, 3), tuning=tuning))
-    if not args.no_cpu:
+    if summary or not args.no_cpu:
         got = out_d.cpu().numpy()
         ref = ops.sddmm_fp64(A, X, Y)
         out["max_rel_err_vs_fp64"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    if not summary and not args.no_cpu:
         rows = np.repeat(np.arange(M), np.diff(A.indptr))
         s = min(A.nnz, 1 << 18)
 
@@ -646,6 +659,8 @@ def main():
     ap.add_argument("--no-n128", action="store_true", help="pruned layer: skip the N=128 secondary figures")
     ap.add_argument("--no-batch", action="store_true",
                     help="pruned layer: skip the 120-matrix encoder batch sub-object")
+    ap.add_argument("--no-small", action="store_true",
+                    help="pruned layer: skip the uniform SpMM and R-MAT SDDMM sub-objects")
     ap.add_argument("--plan-cache", default=None,
                     help="directory of tuned packs: used if present (no search), written otherwise")
     args = ap.parse_args()
