@@ -20,7 +20,9 @@ Other workloads (--workload), each printed as its own JSON line:
   rmat_sddmm    R-MAT scale 16 (+ self loops), X, Y 65536 x 32 FP32
   gcn           GCN layer on R-MAT scale 20 (32.45M nonzeros, symmetric
                 normalised), H 1M x 128, W 128 x 128 FP32: forward A.(HW),
-                backward A^T.dY and SDDMM dA (one step = forward + backward)
+                backward A^T.dY and SDDMM dA (one step = forward + backward);
+                sub-object bf16: the forward product with H.W in BF16 and
+                BF16 blocks (the configuration's BF16 variant)
   attention     64 sparse-attention instances, L ~ U[512, 8192], d = 64,
                 band 128 + 16 random columns: softmax(QK^T / 8 on the mask) V
                 in one fused kernel, replayed as a HIP graph (one step = one
@@ -554,7 +556,40 @@ def run_gcn(args, dev):
         out["max_rel_err_vs_fp64_sample"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
         HWh = np.ascontiguousarray(HW.cpu().numpy())
         out["cpu_baseline"] = cpu_spmm_baseline(A, HWh, 16, 10.0, f"{M}x{M}")
+    out["bf16"] = gcn_bf16_forward(args, dev, A, HW)
     return out
+
+
+def gcn_bf16_forward(args, dev, A, HW):
+    """The configuration's BF16 variant (H, W in BF16): the forward product
+    A.(HW) with H.W in BF16 as B and BF16 MFMA blocks (FP32 sparse products,
+    accumulation and C), its own tuned plan; error against FP64 on B's BF16
+    values, sampled rows."""
+    import numpy as np
+    import torch
+    from stile_amd import api
+    M, F = HW.shape
+    t0 = time.perf_counter()
+    plan, _, tuning = api.tune_plan("spmm", A, "bfloat16", F, log=progress, b_dtype="bfloat16", workers=8,
+                                    cache=cache_path(args, f"gcn{args.gcn_scale}_bf16"))
+    t_plan = time.perf_counter() - t0
+    HWb = HW.to(torch.bfloat16)
+    C = torch.empty_like(HW)
+    steps = max(args.steps // 20, 5)
+    ms = time_launches(lambda: plan.run_spmm(HWb, C), steps, 2)
+    rows = np.random.default_rng(0).choice(M, 2000, replace=False)
+    plan.run_spmm(HWb, C)
+    got = C[torch.from_numpy(rows).to(dev)].double().cpu().numpy()
+    ref = A[rows].astype(np.float64) @ HWb.double().cpu().numpy()
+    info = plan.info()
+    plan.close()
+    return {"spmm_ms": round(ms, 4), "gflops": _gf(2 * A.nnz * F, ms),
+            "roofline": roofline(spmm_alg_bytes(A.nnz, M, M, F, s_b=2), ms),
+            "max_rel_err_vs_fp64_sample": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))),
+            "plan": {"windows": info["num_windows"], "mfma_chunks": info["num_chunks"],
+                     "block_nnz": info["block_nnz"], "sparse_nnz": info["sparse_nnz"],
+                     "search_tune_and_plan_s": round(t_plan, 3),
+                     "tuning": {k: tuning.get(k) for k in ("choice", "ms", "build")}}}
 
 
 def run_attention(args, dev):
