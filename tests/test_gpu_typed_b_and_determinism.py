@@ -119,12 +119,14 @@ def _long_rows(seed):
     return scipy.sparse.csr_matrix((vals, (rows, cols)), shape=(M, K)), lens
 
 
-@pytest.mark.parametrize("N", [64, 72])
+@pytest.mark.parametrize("N", [64, 72, 200])
 def test_deterministic_split_rows(cuda, N):
     """Rows of up to 5000 sparse nonzeros (pieces of 256): the default mode
     adds the pieces with atomics; the deterministic mode sums them through
     the workspace in piece order.  Both within the FP32 bound; the
-    deterministic runs bit-identical; a too-small workspace is refused."""
+    deterministic runs bit-identical; a too-small workspace is refused.
+    N = 200: four column chunks, the last partial (the sparse and fix-up
+    kernels take their waves column chunk by column chunk)."""
     import torch
     A, lens = _long_rows(41)
     B = np.random.default_rng(42).standard_normal((A.shape[1], N), dtype=np.float32)
