@@ -1073,7 +1073,7 @@ __device__ __forceinline__ void store_window_lds(float* __restrict__ C, int64_t 
   }
 }
 
-template <typename TB, bool kMixed, bool kVec>
+template <typename TB, bool kMixed, bool kVec, bool kSc>
 __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __restrict__ tc_ptr,
                                          const int32_t* __restrict__ sp_off, const uint32_t* __restrict__ tc_kind,
                                          const int32_t* __restrict__ win_r0, const int4* __restrict__ units,
@@ -1137,8 +1137,10 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
     }
     return sr;
   };
-  // chunk kind (scatter or dense): a scalar load of its word of tc_kind
-  auto is_sc = [&](int idx) { return tc_kind && ((tc_kind[idx >> 5] >> (idx & 31)) & 1u) != 0; };
+  // chunk kind (scatter or dense): kSc (the plan has scatter chunks, tc_kind
+  // set) -- a scalar load of its word of tc_kind; otherwise dense, with no
+  // per-chunk bookkeeping (measured 4 % of the block-pruned layer's launch)
+  auto is_sc = [&](int idx) { return kSc && ((tc_kind[idx >> 5] >> (idx & 31)) & 1u) != 0; };
   int w = w_beg;
   int w_next = 0;
   SlotRows wr{};
@@ -1266,10 +1268,16 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   extern __shared__ uint4 lds_dyn[];  // [ksz][kLdsRowBytes]
   lds_u8* lds = (lds_u8*)lds_dyn;
   const int tb = (int)gridDim.x - 1 - (int)blockIdx.x;
+  auto run = [&](int item) {  // tc_kind: kernel-uniform
+    if (tc_kind)
+      lds_item<TB, kMixed, kVec, true>(p, tc_ptr, sp_off, tc_kind, win_r0, units, B, ldb, N, C, ldc, nC, vec, vec16,
+                                       cbytes, item, lds);
+    else
+      lds_item<TB, kMixed, kVec, false>(p, tc_ptr, sp_off, tc_kind, win_r0, units, B, ldb, N, C, ldc, nC, vec,
+                                        vec16, cbytes, item, lds);
+  };
   if (!kLoop) {
-    if ((int)blockIdx.x < n_items)
-      lds_item<TB, kMixed, kVec>(p, tc_ptr, sp_off, tc_kind, win_r0, units, B, ldb, N, C, ldc, nC, vec, vec16,
-                                 cbytes, blockIdx.x, lds);
+    if ((int)blockIdx.x < n_items) run(blockIdx.x);
     if (tb < tail_blocks) {
       const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
       if (task < tail_tasks) sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, task);
@@ -1292,8 +1300,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   }
   for (int item = beg; item < end; item += step) {
     __syncthreads();  // every wave is done reading the previous item's block
-    lds_item<TB, kMixed, kVec>(p, tc_ptr, sp_off, tc_kind, win_r0, units, B, ldb, N, C, ldc, nC, vec, vec16, cbytes,
-                               item, lds);
+    run(item);
   }
   if (tb < tail_blocks) {
     const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
