@@ -1119,31 +1119,23 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   // so three are kept in flight.  Four register sets rotate through a loop
   // unrolled by four, so no set is copied (a copy would wait for the newest
   // load).  Every window of the run holds at least one chunk, so at most one
-  // window ends before each chunk.  The first window's rows and the first
-  // two chunks' operands are requested before the B block is staged, so
-  // their latency overlaps the staging, the third one after it (fewer
-  // registers live while the staging holds its loads).
+  // window ends before each chunk.  The first two chunks' operands are
+  // requested before the B block is staged, so their latency overlaps the
+  // staging, the third one after it (fewer registers live while the staging
+  // holds its loads).
+  // A window's rows are not carried in registers: for consecutive rows they
+  // are computed when the window is stored, from its first row (r0w, a
+  // scalar load); other windows load theirs then.  (Rows loaded ahead into
+  // registers that the consecutive path overwrites made every window end
+  // wait for all loads in flight, the next chunks' operands included.)
   const int last = c_end - 1;
-  // a window's rows: computed from its first row (a scalar load) when they
-  // are consecutive, else loaded
-  auto rows_of = [&](int win) {
-    const int r0 = win_r0[win];
-    SlotRows sr;
-    if (r0 >= 0) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) sr.row[i] = r0 + 4 * g + i;
-    } else {
-      sr = load_win(p, win * 16 + g * 4);
-    }
-    return sr;
-  };
   // chunk kind (scatter or dense): kSc (the plan has scatter chunks, tc_kind
   // set) -- a scalar load of its word of tc_kind; otherwise dense, with no
   // per-chunk bookkeeping (measured 4 % of the block-pruned layer's launch)
   auto is_sc = [&](int idx) { return kSc && ((tc_kind[idx >> 5] >> (idx & 31)) & 1u) != 0; };
   int w = w_beg;
   int w_next = 0;
-  SlotRows wr{};
+  int r0w = 0;
   TrChunk r0{}, r1{}, r2{};
   // kMixed: sp_off of the current window and the next, and the lane's
   // entries of the current window's rounds 0 .. kPfRounds - 1, requested when
@@ -1161,7 +1153,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   };
   if (has) {
     w_next = tc_ptr[w + 1];
-    wr = rows_of(w);
+    r0w = win_r0[w];
     r0 = load_tr_chunk(p, c_beg, cidx, lane, is_sc(c_beg));
     r1 = load_tr_chunk(p, min(c_beg + 1, last), cidx, lane, is_sc(min(c_beg + 1, last)));
     if constexpr (kMixed) {
@@ -1177,7 +1169,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   f32x4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  auto finish = [&](const SlotRows& cur) {
+  auto finish = [&]() {
     if constexpr (kMixed) {
       const int rounds = (so1 - so0) >> 4;  // wave-uniform
       if (rounds > 0) {
@@ -1197,6 +1189,18 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
         }
       }
     }
+    SlotRows cur;
+    if (r0w >= 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cur.row[i] = r0w + 4 * g + i;
+    } else {
+      // The loaded rows are consumed inside this branch (the empty asm reads
+      // them), so the wait for the load (and for the loads in flight before
+      // it) stays on this path: where the branches join no load is pending
+      // on the registers the consecutive path writes.
+      cur = load_win(p, w * 16 + g * 4);
+      asm volatile("" : "+v"(cur.row[0]), "+v"(cur.row[1]), "+v"(cur.row[2]), "+v"(cur.row[3]));
+    }
     store_window_lds<kVec>(C, ldc, N, col, vec, cbytes, cur, acc);
   };
   TrChunk r3;
@@ -1204,19 +1208,16 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   auto step = [&](const TrChunk& use, TrChunk& fill) {
     fill = load_tr_chunk(p, min(c + 3, last), cidx, lane, is_sc(min(c + 3, last)));
     if (c >= w_next) {  // window w is complete: store it
-      // The next window's rows are loaded into wr before the stores: a copy
-      // of a load result after the (conditional) stores would make the wave
-      // wait for every store it just issued.
-      const SlotRows cur = wr;
-      wr = rows_of(w + 1);
       const int w_next2 = tc_ptr[w + 2];
+      const int r0n = win_r0[w + 1];
       int so2 = 0;
       if constexpr (kMixed) so2 = sp_off[w + 2];
-      finish(cur);
+      finish();
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
       ++w;
       w_next = w_next2;
+      r0w = r0n;
       if constexpr (kMixed) {
         so0 = so1;
         so1 = so2;
@@ -1235,7 +1236,7 @@ __device__ __forceinline__ void lds_item(const SpmmPlanDev& p, const int32_t* __
   if (c < c_end) step(r0, r3);
   if (c < c_end) step(r1, r0);
   if (c < c_end) step(r2, r1);
-  finish(wr);
+  finish();
 }
 
 // units: the work-unit list of this launch.  kLoop = false: one workgroup
