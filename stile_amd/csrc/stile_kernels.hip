@@ -71,19 +71,22 @@ __device__ __forceinline__ float4 cvt4(uint2 w) {
   }
 }
 
+// B[k][col .. col + 3] as float4, zero at or beyond N.  vec (N % 4 == 0,
+// rows aligned): one unconditional load from a column clamped into the row
+// and a select after it.  (A load under an exec mask leaves its registers
+// partly unwritten, and the compiler then waits for every load in flight
+// before it reuses them: in a loop of gathers that serialises the gathers.)
 template <typename TB>
 __device__ __forceinline__ float4 load_b4(const TB* __restrict__ B, int64_t ldb, int k, int col, int N, bool vec) {
   const TB* p = B + (int64_t)k * ldb + col;
-  if constexpr (std::is_same<TB, float>::value) {
-    if (vec) {
-      if (col < N) return ld4(p);
-      return make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-  } else {
-    if (vec) {
-      if (col < N) return cvt4<TB>(*reinterpret_cast<const uint2*>(p));
-      return make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+  if (vec) {
+    const TB* q = B + (int64_t)k * ldb + min(col, N - 4);
+    float4 r;
+    if constexpr (std::is_same<TB, float>::value)
+      r = ld4(q);
+    else
+      r = cvt4<TB>(*reinterpret_cast<const uint2*>(q));
+    return col < N ? r : make_float4(0.f, 0.f, 0.f, 0.f);
   }
   float4 r;
   r.x = col < N ? (float)p[0] : 0.f;
@@ -91,6 +94,19 @@ __device__ __forceinline__ float4 load_b4(const TB* __restrict__ B, int64_t ldb,
   r.z = col + 2 < N ? (float)p[2] : 0.f;
   r.w = col + 3 < N ? (float)p[3] : 0.f;
   return r;
+}
+
+// f(std::true_type) when vec, else f(std::false_type): the row gathers of
+// the sparse kernels see vec as a constant, so the 16-byte and the
+// per-element paths of load_b4 are not merged into one loop, where the
+// per-element path's masked loads make every gather wait for the ones
+// before it.
+template <typename F>
+__device__ __forceinline__ void with_vec(bool vec, F&& f) {
+  if (vec)
+    f(std::true_type{});
+  else
+    f(std::false_type{});
 }
 
 // NT: non-temporal store (streamed past the caches; for rows nothing reads
@@ -727,7 +743,7 @@ __global__ __launch_bounds__(256, 7) void hyb_spmm_sparse_kernel(SpmmPlanDev p, 
     const int q = (int)(wave - cch * per);
     const int col = cch * 64 + 4 * (lane & 15);
     const int m = q * 4 + (lane >> 4);
-    if (m < n_mix) add_slot(p, B, ldb, N, C, ldc, col, vec, p.mix_slot[m]);
+    if (m < n_mix) with_vec(vec, [&](auto V) { add_slot(p, B, ldb, N, C, ldc, col, V, p.mix_slot[m]); });
     return;
   }
   const int64_t idx = wave - mix_waves;
@@ -739,7 +755,7 @@ __global__ __launch_bounds__(256, 7) void hyb_spmm_sparse_kernel(SpmmPlanDev p, 
   const int quarter = (int)(rr & 3);
   const int n0 = cch * 64;
   const int col = n0 + 4 * (lane & 15);
-  finish_slot(p, B, ldb, N, C, ldc, col, vec, w * 16 + quarter * 4 + (lane >> 4));
+  with_vec(vec, [&](auto V) { finish_slot(p, B, ldb, N, C, ldc, col, V, w * 16 + quarter * 4 + (lane >> 4)); });
 }
 
 // Sparse-only rows for launches with too few rows to fill the GPU at one row
@@ -816,7 +832,9 @@ template <typename TB>
 __global__ __launch_bounds__(256) void hyb_spmm_sparse_split_kernel(SpmmPlanDev p, const TB* __restrict__ B,
                                                                     int64_t ldb, int N, float* __restrict__ C,
                                                                     int64_t ldc, int nC, bool vec) {
-  sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
+  with_vec(vec, [&](auto V) {
+    sparse_split_wave(p, B, ldb, N, C, ldc, nC, V, (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6));
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1281,7 +1299,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
     if ((int)blockIdx.x < n_items) run(blockIdx.x);
     if (tb < tail_blocks) {
       const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
-      if (task < tail_tasks) sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, task);
+      if (task < tail_tasks) with_vec(vec, [&](auto V) { sparse_split_wave(p, B, ldb, N, C, ldc, nC, V, task); });
     }
     return;
   }
@@ -1305,7 +1323,7 @@ __global__ __launch_bounds__(kLdsThreads) void hyb_spmm_lds_bf16_kernel(SpmmPlan
   }
   if (tb < tail_blocks) {
     const int task = tb * (kLdsThreads / 64) + (int)(threadIdx.x >> 6);
-    if (task < tail_tasks) sparse_split_wave(p, B, ldb, N, C, ldc, nC, vec, task);
+    if (task < tail_tasks) with_vec(vec, [&](auto V) { sparse_split_wave(p, B, ldb, N, C, ldc, nC, V, task); });
   }
 }
 
@@ -1333,6 +1351,7 @@ __global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, cons
   const int g = lane >> 4;
   const int col = n0 + 4 * (lane & 15);
   const int beg = p.piece_beg[pc], end = p.piece_end[pc];
+  with_vec(vec, [&](auto V) {
   float4 s = make_float4(0.f, 0.f, 0.f, 0.f), cc = make_float4(0.f, 0.f, 0.f, 0.f);
   // eight gathers in flight per lane group (positions past the end read the
   // piece's last entry with weight 0)
@@ -1348,7 +1367,7 @@ __global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, cons
     }
     float4 b[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) b[i] = load_b4(B, ldb, k[i], col, N, vec);
+    for (int i = 0; i < 8; ++i) b[i] = load_b4(B, ldb, k[i], col, N, V);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       neu_add(s.x, cc.x, v[i] * b[i].x);
@@ -1367,7 +1386,7 @@ __global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, cons
   }
   if (g != 0) return;
   if (ws) {
-    store_c4(ws, N, pc, col, N, vec, s);
+    store_c4(ws, N, pc, col, N, V, s);
     return;
   }
   float* cp = C + (int64_t)p.piece_row[pc] * ldc + col;
@@ -1375,6 +1394,7 @@ __global__ __launch_bounds__(256) void hyb_spmm_fixup_kernel(SpmmPlanDev p, cons
   if (col + 1 < N) atomicAdd(cp + 1, s.y);
   if (col + 2 < N) atomicAdd(cp + 2, s.z);
   if (col + 3 < N) atomicAdd(cp + 3, s.w);
+  });
 }
 
 // Deterministic mode: C[row] += the row's pieces in piece order, for split
