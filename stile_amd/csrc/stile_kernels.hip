@@ -648,7 +648,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p,
     f32x4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    window_dense<MODE>(p, B, ldb, N, col, vec, a, b, lane, acc);
+    with_vec(vec, [&](auto V) { window_dense<MODE>(p, B, ldb, N, col, V, a, b, lane, acc); });
     if (wv > 0) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) part[wv - 1][t][lane] = acc[t];
@@ -685,7 +685,8 @@ __global__ __launch_bounds__(kBigWaves * 64) void hyb_spmm_kernel(SpmmPlanDev p,
   const int64_t wave = (int64_t)(blockIdx.x - huge_blocks - big_blocks) * kBigWaves + wv;
   if (wave >= (int64_t)(p.W_tc - p.W_big) * nC) return;
   const int w = p.W_big + (int)(wave / nC);
-  hyb_spmm_window<MODE>(p, B, ldb, N, C, ldc, w, (int)(wave - (int64_t)(w - p.W_big) * nC) * 64, vec);
+  const int n0 = (int)(wave - (int64_t)(w - p.W_big) * nC) * 64;
+  with_vec(vec, [&](auto V) { hyb_spmm_window<MODE>(p, B, ldb, N, C, ldc, w, n0, V); });
 }
 
 // Deterministic mode: C rows of huge window w = sum of its parts' partial
