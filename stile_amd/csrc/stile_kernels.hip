@@ -1681,17 +1681,23 @@ __global__ __launch_bounds__(256, 8) void hyb_sddmm_kernel(SddmmPlanDev p,
 #pragma unroll
     for (int u = 0; u < kSddmmU; ++u) s[u] = 0.f;
     if (vec) {
+      // consecutive entries of one row (CSR order) share the X load; an
+      // entry whose row repeats skips its load and takes the previous X
+      // value in a select after every load is issued (a copy inside the
+      // branch would wait for the previous load before the next one)
+      bool same[kSddmmU];
+#pragma unroll
+      for (int u = 0; u < kSddmmU; ++u) same[u] = u > 0 && row[u] == row[u - 1];
       for (int k = 4 * sub; k < Kf; k += 4 * LPE) {
         float4 a[kSddmmU], b[kSddmmU];
 #pragma unroll
         for (int u = 0; u < kSddmmU; ++u) {
-          // consecutive entries of one row (CSR order) share the X load
-          if (u > 0 && row[u] == row[u - 1])
-            a[u] = a[u - 1];
-          else
-            a[u] = ld4(X + (int64_t)row[u] * ldx + k);
+          if (!same[u]) a[u] = ld4(X + (int64_t)row[u] * ldx + k);
           b[u] = ld4(Y + (int64_t)col[u] * ldy + k);
         }
+#pragma unroll
+        for (int u = 1; u < kSddmmU; ++u)
+          if (same[u]) a[u] = a[u - 1];
 #pragma unroll
         for (int u = 0; u < kSddmmU; ++u) s[u] += a[u].x * b[u].x + a[u].y * b[u].y + a[u].z * b[u].z + a[u].w * b[u].w;
       }
