@@ -466,7 +466,8 @@ def run_rmat_sddmm(args, dev, summary=False):
     out = line("SDDMM GFLOP/s (2*nnz*K)", flops / (ms * 1e-3) / 1e9, "GFLOP/s", ms, args, "f32",
                {"workload": f"SDDMM FP32, R-MAT scale 16 + self loops (nnz={A.nnz}), X, Y {M}x{Kf}",
                 "M": M, "N": Nc, "K": Kf, "nnz": int(A.nnz), "parallelism": "single GPU"},
-               roofline=roofline(sddmm_alg_bytes(A.nnz, M, Nc, Kf), ms),
+               roofline=roofline(sddmm_alg_bytes(A.nnz, M, Nc, Kf), ms,
+                                 measured_traffic("hyb_sddmm_traffic_rmat_sddmm.json")),
                plain_csr_gflops=_gf(flops, ms_csr), hybrid_over_csr=round(ms_csr / ms, 3),
                plan=dict(plan.info(), search_tune_and_plan_s=round(t_plan, 3), tuning=tuning))
     if summary or not args.no_cpu:
